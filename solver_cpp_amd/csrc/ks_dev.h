@@ -1,0 +1,340 @@
+// ks_dev.h - host-side infrastructure of libksolve: device buffers, the
+// context (stream, reduction scratch, statistics), the matrix (CSR in HBM,
+// FP32 value copy, device-built transpose) and the kernel launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/ksolve.h"
+#include "ks_kernels.h"
+
+namespace ks {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define KS_HIPCK(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::ks::Error(KS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_) +     \
+                                        " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define KS_CHECK(cond, status, msg)                     \
+  do {                                                  \
+    if (!(cond)) throw ::ks::Error((status), (msg));    \
+  } while (0)
+
+// Owning device array.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t m) { alloc(m); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t m) {
+    release();
+    if (m == 0) return;
+    hipError_t e = hipMalloc((void**)&p, m * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw Error(KS_ERR_NO_MEMORY, "hipMalloc of " + std::to_string(m * sizeof(T)) + " bytes failed: " +
+                                        hipGetErrorString(e));
+    }
+    n = m;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Per-class hipEvent timing of launches (profile mode).
+struct Prof {
+  unsigned mask = 0;
+  struct Pair { hipEvent_t a, b; int cls; };
+  std::vector<hipEvent_t> pool;
+  std::vector<Pair> pend;
+  size_t used = 0;
+  int cur = -1;
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      KS_HIPCK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void begin(hipStream_t s, int cls) {
+    cur = -1;
+    if (!(mask & (1u << cls))) return;
+    Pair pr{get(), get(), cls};
+    KS_HIPCK(hipEventRecord(pr.a, s));
+    pend.push_back(pr);
+    cur = cls;
+  }
+  void end(hipStream_t s) {
+    if (cur < 0) return;
+    KS_HIPCK(hipEventRecord(pend.back().b, s));
+    cur = -1;
+  }
+  // after the stream is synchronised
+  void collect(double* ms) {
+    for (auto& pr : pend) {
+      float t = 0.f;
+      KS_HIPCK(hipEventElapsedTime(&t, pr.a, pr.b));
+      ms[pr.cls] += t;
+    }
+    pend.clear();
+    used = 0;
+  }
+  void destroy() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+    pend.clear();
+    used = 0;
+  }
+};
+
+}  // namespace ks
+
+struct ks_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  std::string err;
+  ks::DBuf<double> part;                 // kMaxGrid * kMaxNV reduction partials
+  ks::DBuf<unsigned> counter;            // reduction ticket
+  ks::DBuf<unsigned long long> stats;    // launches[8], bytes[8]
+  ks::DBuf<double> scal;                 // scalar outputs of the BLAS entry points
+  int* hflags = nullptr;                 // host-mapped flags (host view)
+  int* hflags_dev = nullptr;             // same memory, device view
+  int num_cus = 256;
+  ks::Prof prof;
+  std::vector<hipEvent_t> evpool;
+};
+
+namespace ks {
+
+struct Plan {
+  int L = 8;  // lanes per row
+};
+
+template <class T>
+struct MatView {
+  int64_t n = 0, nnz = 0;
+  const int32_t* ptr = nullptr;
+  const int32_t* col = nullptr;
+  const T* val = nullptr;
+  int L = 8;
+};
+
+}  // namespace ks
+
+struct ks_mat {
+  ks_ctx* ctx = nullptr;
+  int64_t n = 0, nnz = 0;
+  uint32_t flags = 0;
+  ks::DBuf<int32_t> ptr, col;
+  ks::DBuf<double> val;
+  ks::DBuf<float> val32;
+  bool has_t = false;
+  ks::DBuf<int32_t> tptr, tcol, tperm;
+  ks::DBuf<double> tval;
+  ks::DBuf<float> tval32;
+  ks::Plan plan, tplan;
+  int64_t max_row = 0, max_trow = 0;
+
+  template <class T>
+  ks::MatView<T> view(bool trans) const;
+};
+
+template <>
+inline ks::MatView<double> ks_mat::view<double>(bool trans) const {
+  ks::MatView<double> v;
+  v.n = n; v.nnz = nnz;
+  if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval.p; v.L = tplan.L; }
+  else { v.ptr = ptr.p; v.col = col.p; v.val = val.p; v.L = plan.L; }
+  return v;
+}
+template <>
+inline ks::MatView<float> ks_mat::view<float>(bool trans) const {
+  ks::MatView<float> v;
+  v.n = n; v.nnz = nnz;
+  if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval32.p; v.L = tplan.L; }
+  else { v.ptr = ptr.p; v.col = col.p; v.val = val32.p; v.L = plan.L; }
+  return v;
+}
+
+namespace ks {
+
+inline RedArgs red(ks_ctx* c) { return RedArgs{c->part.p, c->counter.p}; }
+inline StatArgs stat(ks_ctx* c, int cls, double bytes) {
+  return StatArgs{c->stats.p, c->stats.p + KS_K_NCLASS, cls,
+                  (unsigned long long)(bytes > 0 ? bytes : 0)};
+}
+
+inline int grid_map(int64_t n) {
+  int64_t g = (n + (int64_t)kBlock * 4 - 1) / ((int64_t)kBlock * 4);
+  if (g < 1) g = 1;
+  if (g > kMaxGrid) g = kMaxGrid;
+  if (g >= 64) g &= ~7;
+  return (int)g;
+}
+inline int grid_rows(int64_t nrb) {
+  int64_t g = nrb < kMaxGrid ? nrb : kMaxGrid;
+  if (g < 1) g = 1;
+  if (g >= 64) g &= ~7;
+  return (int)g;
+}
+
+template <int NV, class Op>
+void launch_map(ks_ctx* c, int64_t n, const Op& op, Gate g, int cls, double bytes) {
+  const int G = grid_map(n);
+  c->prof.begin(c->stream, cls);
+  k_map<NV, Op><<<G, kBlock, 0, c->stream>>>(n, op, g, red(c), stat(c, cls, bytes));
+  c->prof.end(c->stream);
+  KS_HIPCK(hipGetLastError());
+}
+
+template <class Op>
+void launch_scalar(ks_ctx* c, const Op& op, Gate g) {
+  k_scalar<Op><<<1, 1, 0, c->stream>>>(op, g, stat(c, KS_K_SCALAR, 0));
+  KS_HIPCK(hipGetLastError());
+}
+
+// SpMV bytes: values + column indices + row pointers + x read once + y written.
+template <class T>
+inline double spmv_bytes(int64_t n, int64_t nnz) {
+  return (double)(sizeof(T) + 4) * nnz + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
+}
+
+template <class T, int L, int NV, class Epi>
+void spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, int cls, double bytes) {
+  constexpr int RPB = kBlock / L;
+  const int64_t nrb = (A.n + RPB - 1) / RPB;
+  const int G = grid_rows(nrb);
+  c->prof.begin(c->stream, cls);
+  k_spmv<T, L, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ptr, A.col, A.val, x, epi, g, red(c),
+                                                     stat(c, cls, bytes));
+  c->prof.end(c->stream);
+  KS_HIPCK(hipGetLastError());
+}
+
+// y-side epilogue of SpMV plus NV reduction values.
+template <class T, int NV, class Epi>
+void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, double extra_bytes = 0) {
+  const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
+  const double bytes = spmv_bytes<T>(A.n, A.nnz) + extra_bytes;
+  switch (A.L) {
+    case 2: spmv_L<T, 2, NV>(c, A, x, epi, g, cls, bytes); break;
+    case 4: spmv_L<T, 4, NV>(c, A, x, epi, g, cls, bytes); break;
+    case 8: spmv_L<T, 8, NV>(c, A, x, epi, g, cls, bytes); break;
+    case 16: spmv_L<T, 16, NV>(c, A, x, epi, g, cls, bytes); break;
+    case 32: spmv_L<T, 32, NV>(c, A, x, epi, g, cls, bytes); break;
+    default: spmv_L<T, 64, NV>(c, A, x, epi, g, cls, bytes); break;
+  }
+}
+
+template <class T, int L>
+void spmv2_L(ks_ctx* c, const MatView<T>& A, const T* x1, const T* x2, T* y1, T* y2, Gate g) {
+  constexpr int RPB = kBlock / L;
+  const int64_t nrb = (A.n + RPB - 1) / RPB;
+  const int G = grid_rows(nrb);
+  const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * A.n;
+  c->prof.begin(c->stream, KS_K_BASIS);
+  k_spmv2<T, L><<<G, kBlock, 0, c->stream>>>(A.n, A.ptr, A.col, A.val, x1, x2, y1, y2, g,
+                                            stat(c, KS_K_BASIS, bytes));
+  c->prof.end(c->stream);
+  KS_HIPCK(hipGetLastError());
+}
+template <class T>
+void launch_spmv2(ks_ctx* c, const MatView<T>& A, const T* x1, const T* x2, T* y1, T* y2, Gate g) {
+  switch (A.L) {
+    case 2: spmv2_L<T, 2>(c, A, x1, x2, y1, y2, g); break;
+    case 4: spmv2_L<T, 4>(c, A, x1, x2, y1, y2, g); break;
+    case 8: spmv2_L<T, 8>(c, A, x1, x2, y1, y2, g); break;
+    case 16: spmv2_L<T, 16>(c, A, x1, x2, y1, y2, g); break;
+    case 32: spmv2_L<T, 32>(c, A, x1, x2, y1, y2, g); break;
+    default: spmv2_L<T, 64>(c, A, x1, x2, y1, y2, g); break;
+  }
+}
+
+// Lanes per row: smallest power of two >= mean row length, in [2, 64].
+inline int choose_lanes(int64_t n, int64_t nnz) {
+  const double mean = n > 0 ? (double)nnz / (double)n : 1.0;
+  int L = 2;
+  while (L < 64 && L < mean) L *= 2;
+  return L;
+}
+
+// ---------------------------------------------------------- epilogues ---
+template <class T>
+struct EpiStore {
+  T* y;
+  __device__ void prep() {}
+  __device__ void row(int64_t i, T s, double*) { y[i] = s; }
+  __device__ void fin(const double*) {}
+};
+
+// y = b - A x (b FP64), acc += y^2, Fin on the total.
+template <class T, class Fin>
+struct EpiResid {
+  const double* b;
+  T* y;
+  Fin f;
+  __device__ void prep() {}
+  __device__ void row(int64_t i, T s, double* acc) {
+    const T r = (T)(b[i] - (double)s);
+    y[i] = r;
+    acc[0] += (double)r * (double)r;
+  }
+  __device__ void fin(const double* s) { f(s); }
+};
+
+// y = A x, acc += d . y, Fin on the total.
+template <class T, class Fin>
+struct EpiDot {
+  const T* d;
+  T* y;
+  Fin f;
+  __device__ void prep() {}
+  __device__ void row(int64_t i, T s, double* acc) {
+    y[i] = s;
+    acc[0] += (double)d[i] * (double)s;
+  }
+  __device__ void fin(const double* s) { f(s); }
+};
+
+struct FinNone {
+  __device__ void operator()(const double*) const {}
+};
+// writes sum (or its square root) to a device scalar
+struct FinStore {
+  double* out;
+  int sqrt_it;
+  __device__ void operator()(const double* s) const { *out = sqrt_it ? sqrt(s[0]) : s[0]; }
+};
+
+}  // namespace ks

@@ -1,0 +1,696 @@
+// ks_kernels.h - HIP kernels for gfx950 (CDNA4, wave64).
+//
+// Every kernel here is HBM-bound: the CSR SpMV reads (s_v+4)*nnz bytes of
+// matrix per call, the BLAS-1 / Gram-Schmidt kernels stream whole vectors.
+// Design points:
+//  * one generic grid-stride map/reduce kernel (k_map) carries every fused
+//    elementwise update + dot product of the solvers; reductions are
+//    deterministic two-stage sums (fixed grid, fixed per-thread element
+//    sets, fixed shuffle tree, the last workgroup sums the per-workgroup
+//    partials in workgroup order);
+//  * CSR SpMV (k_spmv) assigns L lanes per row (L = 2..64, 64 = one
+//    wavefront per row), stages each workgroup's row pointers in LDS and
+//    reduces a row's lane sums with xor shuffles; row blocks are assigned to
+//    XCDs in contiguous ranges so x gathers hit the XCD's own L2;
+//  * solver control flow stays on the device: kernels take a Gate and return
+//    at once when the solve (or the enclosing outer step) has stopped, so
+//    the host can enqueue many iterations between synchronisations;
+//  * scalar recurrences (alpha, beta, Givens rotations, k-skip updates) run
+//    in the finaliser of the reduction that produces their inputs, or in a
+//    one-thread kernel, in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ks {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wavefronts)
+constexpr int kMaxGrid = 2048;    // workgroups of a reduction grid (8/CU at 256 CUs)
+constexpr int kMaxNV = 64;        // reduction values per kernel
+constexpr int kXcd = 8;           // XCDs per MI355X; workgroup b runs on XCD b % 8
+
+// ---------------------------------------------------------------- control --
+// Device-resident state of one solve level (outer or inner).
+struct Ctl {
+  int stop;        // 1: no further Krylov steps (converged / limit / breakdown)
+  int done;        // 1: final assembly done (GMRES family)
+  int status;      // 0 converged, 2 not converged, 3 breakdown
+  int ksz;         // GMRES: steps completed in the cycle that stopped
+  long long count; // iteration counter (meaning per solver)
+  long long nhist;
+  long long hist_cap;
+  long long maxloop;
+  long long steps;
+  long long tsteps;  // steps over all solves of this level (never reset)
+  double eps, bnorm, err;
+  double rr, alpha, beta, gamma, dotv, vscale, rz;
+  double* hist;    // (idx, err) pairs, device
+  int* host_stop;  // host-mapped flag mirror of `stop` (nullable)
+  int R, ld;       // GMRES restart and H row stride
+  double *H, *e, *cs, *sn, *y;
+  int kskip, fix, compat, pad;
+  double *delta, *eta, *zeta, *rho, *phi;
+};
+
+__device__ inline void ctl_stop(Ctl* c, int status) {
+  c->stop = 1;
+  c->status = status;
+  if (c->host_stop) *(volatile int*)c->host_stop = 1;
+}
+__device__ inline void ctl_hist(Ctl* c, double idx, double err) {
+  c->err = err;
+  if (c->hist && c->nhist < c->hist_cap) {
+    c->hist[2 * c->nhist] = idx;
+    c->hist[2 * c->nhist + 1] = err;
+  }
+  c->nhist++;
+}
+__device__ inline bool finite(double v) { return __builtin_isfinite(v); }
+
+// A kernel is skipped when either flag is set.
+struct Gate {
+  const int* a;
+  const int* b;
+  __device__ bool off() const { return (a && *a) || (b && *b); }
+};
+
+// Reduction scratch of the context: per-workgroup partials and a ticket.
+struct RedArgs {
+  double* part;
+  unsigned* counter;
+};
+
+// Executed-launch statistics per kernel class (counted by workgroup 0).
+struct StatArgs {
+  unsigned long long* launches;
+  unsigned long long* bytes;
+  int cls;
+  unsigned long long nbytes;
+  __device__ void count() const {
+    if (launches && blockIdx.x == 0 && threadIdx.x == 0) {
+      atomicAdd(launches + cls, 1ull);
+      atomicAdd(bytes + cls, nbytes);
+    }
+  }
+};
+
+// ----------------------------------------------------------- reductions --
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums acc[0..NV) over the grid.  Returns true in the last workgroup to
+// finish, where out[0..NV) (LDS) then holds the totals for every thread.
+// Deterministic: partials are written per workgroup and summed by the last
+// workgroup in a fixed order, independent of completion order.
+template <int NV>
+__device__ bool grid_sum(double (&acc)[NV], RedArgs ra, double* out) {
+  __shared__ double sw[NV][kBlock / 64];
+  __shared__ int s_last;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int nw = kBlock / 64;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double v = wave_sum(acc[i]);
+    if (lane == 0) sw[i][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < nw; ++k) s += sw[i][k];
+      __hip_atomic_store(ra.part + (size_t)blockIdx.x * NV + i, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    unsigned t = atomicAdd(ra.counter, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+      s += __hip_atomic_load(ra.part + (size_t)b * NV + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = wave_sum(s);
+    if (lane == 0) sw[i][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < nw; ++k) s += sw[threadIdx.x][k];
+    out[threadIdx.x] = s;
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(ra.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return true;
+}
+
+// ------------------------------------------------------- generic map ----
+// op.prep() once per thread (load scalars), op.elem(i, acc) per element,
+// op.fin(sums) by one thread of the last workgroup when NV > 0.
+template <int NV, class Op>
+__global__ __launch_bounds__(kBlock) void k_map(int64_t n, Op op, Gate g, RedArgs ra, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  op.prep();
+  double acc[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < (NV > 0 ? NV : 1); ++i) acc[i] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    op.elem(i, acc);
+    op.elem(i + stride, acc);
+    op.elem(i + 2 * stride, acc);
+    op.elem(i + 3 * stride, acc);
+  }
+  for (; i < n; i += stride) op.elem(i, acc);
+  if constexpr (NV > 0) {
+    __shared__ double sums[NV];
+    if (grid_sum<NV>(acc, ra, sums)) {
+      if (threadIdx.x == 0) op.fin(sums);
+    }
+  }
+}
+
+// One-thread kernel for scalar recurrences.
+template <class Op>
+__global__ void k_scalar(Op op, Gate g, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  op.run();
+}
+
+// -------------------------------------------------------------- SpMV ----
+// Row-block r of `nrb` for workgroup slot `it`: XCD-contiguous ranges.
+struct RowBlocks {
+  int64_t nrb;
+  __device__ void range(int64_t& first, int64_t& end, int64_t& step) const {
+    const int G = gridDim.x;
+    if (G % kXcd == 0 && G >= 8 * kXcd) {
+      const int xcd = blockIdx.x % kXcd, bi = blockIdx.x / kXcd, gx = G / kXcd;
+      const int64_t per = (nrb + kXcd - 1) / kXcd;
+      const int64_t lo = (int64_t)xcd * per;
+      int64_t hi = lo + per;
+      if (hi > nrb) hi = nrb;
+      first = lo + bi;
+      end = hi;
+      step = gx;
+    } else {
+      first = blockIdx.x;
+      end = nrb;
+      step = G;
+    }
+  }
+};
+
+// y = A x with L lanes per row (L | 64), RPB = kBlock/L rows per row block.
+// Epi: NV reduction values, prep(), row(row, sum, acc), fin(sums).
+template <class T, int L, int NV, class Epi>
+__global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __restrict__ ptr,
+                                                 const int32_t* __restrict__ col,
+                                                 const T* __restrict__ val, const T* __restrict__ x,
+                                                 Epi epi, Gate g, RedArgs ra, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  epi.prep();
+  constexpr int RPB = kBlock / L;
+  __shared__ int32_t sptr[RPB + 1];
+  double acc[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < (NV > 0 ? NV : 1); ++i) acc[i] = 0.0;
+  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
+  RowBlocks rbs{(n + RPB - 1) / RPB};
+  int64_t rb, end, step;
+  rbs.range(rb, end, step);
+  for (; rb < end; rb += step) {
+    const int64_t row0 = rb * RPB;
+    __syncthreads();
+    if (threadIdx.x <= RPB) {
+      int64_t r = row0 + threadIdx.x;
+      sptr[threadIdx.x] = ptr[r < n ? r : n];
+    }
+    __syncthreads();
+    const int64_t row = row0 + grp;
+    const int s = sptr[grp], e = sptr[grp + 1];
+    T sum = 0;
+    for (int j = s + lane; j < e; j += L) sum += val[j] * x[col[j]];
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
+    if (lane == 0 && row < n) epi.row(row, sum, acc);
+  }
+  if constexpr (NV > 0) {
+    __shared__ double sums[NV];
+    if (grid_sum<NV>(acc, ra, sums)) {
+      if (threadIdx.x == 0) epi.fin(sums);
+    }
+  }
+}
+
+// Two products with one pass over the matrix: y1 = A x1, y2 = A x2
+// (matrix-powers basis of the k-skip solvers).
+template <class T, int L>
+__global__ __launch_bounds__(kBlock) void k_spmv2(int64_t n, const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ col,
+                                                  const T* __restrict__ val, const T* __restrict__ x1,
+                                                  const T* __restrict__ x2, T* __restrict__ y1,
+                                                  T* __restrict__ y2, Gate g, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  constexpr int RPB = kBlock / L;
+  __shared__ int32_t sptr[RPB + 1];
+  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
+  RowBlocks rbs{(n + RPB - 1) / RPB};
+  int64_t rb, end, step;
+  rbs.range(rb, end, step);
+  for (; rb < end; rb += step) {
+    const int64_t row0 = rb * RPB;
+    __syncthreads();
+    if (threadIdx.x <= RPB) {
+      int64_t r = row0 + threadIdx.x;
+      sptr[threadIdx.x] = ptr[r < n ? r : n];
+    }
+    __syncthreads();
+    const int64_t row = row0 + grp;
+    const int s = sptr[grp], e = sptr[grp + 1];
+    T s1 = 0, s2 = 0;
+    for (int j = s + lane; j < e; j += L) {
+      const T a = val[j];
+      const int c = col[j];
+      s1 += a * x1[c];
+      s2 += a * x2[c];
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+      s1 += __shfl_xor(s1, off, L);
+      s2 += __shfl_xor(s2, off, L);
+    }
+    if (lane == 0 && row < n) {
+      y1[row] = s1;
+      y2[row] = s2;
+    }
+  }
+}
+
+// out[j] = alpha * u[row(j)] * v[col(j)] over the pattern (SDDMM).
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_sddmm(int64_t n, const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ col, double alpha,
+                                                  const double* __restrict__ u,
+                                                  const double* __restrict__ v, double* __restrict__ out,
+                                                  StatArgs st) {
+  st.count();
+  constexpr int RPB = kBlock / L;
+  __shared__ int32_t sptr[RPB + 1];
+  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
+  RowBlocks rbs{(n + RPB - 1) / RPB};
+  int64_t rb, end, step;
+  rbs.range(rb, end, step);
+  for (; rb < end; rb += step) {
+    const int64_t row0 = rb * RPB;
+    __syncthreads();
+    if (threadIdx.x <= RPB) {
+      int64_t r = row0 + threadIdx.x;
+      sptr[threadIdx.x] = ptr[r < n ? r : n];
+    }
+    __syncthreads();
+    const int64_t row = row0 + grp;
+    if (row < n) {
+      const double ur = alpha * u[row];
+      const int s = sptr[grp], e = sptr[grp + 1];
+      for (int j = s + lane; j < e; j += L) out[j] = ur * v[col[j]];
+    }
+  }
+}
+
+// ---------------------------------------------------- Gram-Schmidt ------
+// h[i] = (w, V_{i0+i}) for i < nv (nv <= G): one pass over w and nv columns.
+template <class T, int G>
+__global__ __launch_bounds__(kBlock) void k_proj(int64_t n, const T* __restrict__ V, int64_t ldv, int nv,
+                                                 const T* __restrict__ w, double* hout, int ldh, Gate g,
+                                                 RedArgs ra, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  double acc[G];
+#pragma unroll
+  for (int v = 0; v < G; ++v) acc[v] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double wi = (double)w[i];
+#pragma unroll
+    for (int v = 0; v < G; ++v)
+      if (v < nv) acc[v] += wi * (double)V[(int64_t)v * ldv + i];
+  }
+  __shared__ double sums[G];
+  if (grid_sum<G>(acc, ra, sums)) {
+    if (threadIdx.x < nv) hout[(int64_t)threadIdx.x * ldh] = sums[threadIdx.x];
+  }
+}
+
+// w -= sum_{i<nv} h_i V_i (classical GS update), then ||w||^2 -> fin.
+template <class T, class Fin>
+__global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __restrict__ V, int64_t ldv,
+                                                       int nv, T* __restrict__ w, const double* hcol,
+                                                       int ldh, Fin fin, Gate g, RedArgs ra, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  extern __shared__ double sh[];
+  for (int v = threadIdx.x; v < nv; v += kBlock) sh[v] = hcol[(int64_t)v * ldh];
+  __syncthreads();
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    double t = (double)w[i];
+#pragma unroll 8
+    for (int v = 0; v < nv; ++v) t -= sh[v] * (double)V[(int64_t)v * ldv + i];
+    const T tw = (T)t;
+    w[i] = tw;
+    acc[0] += (double)tw * (double)tw;
+  }
+  __shared__ double sums[1];
+  if (grid_sum<1>(acc, ra, sums)) {
+    if (threadIdx.x == 0) fin(sums);
+  }
+}
+
+// x += sum_{i<size} y_i B_i   (or only the last term with KS_COMPAT_LAST_TERM
+// when the cycle stopped inside); y and size come from the Ctl (k_hye).
+// Tx: type of x, Tb: type of the basis B.
+template <class Tx, class Tb>
+__global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __restrict__ B, int64_t ldb,
+                                                     Tx* __restrict__ x, const Ctl* c, int last_term,
+                                                     Gate g, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  extern __shared__ double sy[];
+  __shared__ int s_sz, s_lt;
+  if (threadIdx.x == 0) {
+    s_sz = c->stop ? c->ksz : c->R - 1;
+    s_lt = last_term && c->stop;
+  }
+  __syncthreads();
+  const int sz = s_sz;
+  for (int v = threadIdx.x; v < sz; v += kBlock) sy[v] = c->y[v];
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  if (s_lt) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+      double t = sz > 0 ? sy[sz - 1] * (double)B[(int64_t)(sz - 1) * ldb + i] : 0.0;
+      x[i] = (Tx)((double)x[i] + (double)(Tb)t);
+    }
+  } else {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+      double t = 0.0;
+#pragma unroll 8
+      for (int v = 0; v < sz; ++v) t += sy[v] * (double)B[(int64_t)v * ldb + i];
+      x[i] = (Tx)((double)x[i] + (double)(Tb)t);
+    }
+  }
+}
+
+// ------------------------------------------------------------ GMRES ----
+// Back substitution y = H(0:sz,0:sz)^{-1} e (blas.cpp:303-314).
+__global__ void k_hye(Ctl* c, Gate g) {
+  if (g.off()) return;
+  const int sz = c->stop ? c->ksz : c->R - 1;
+  const int ld = c->ld;
+  for (int i = 0; i < sz; ++i) c->y[i] = 0.0;
+  for (int i = sz - 1; i >= 0; --i) {
+    double t = 0.0;
+    for (int j = i + 1; j < sz; ++j) t += c->y[j] * c->H[i * ld + j];
+    c->y[i] = (c->e[i] - t) / c->H[i * ld + i];
+  }
+}
+
+// Marks the solve done once the stopping cycle has been assembled.
+__global__ void k_mark_done(Ctl* c, Gate g) {
+  if (g.off()) return;
+  if (c->stop) c->done = 1;
+}
+
+// Givens rotations for column k (gmres.cpp:384-401), then the stop test of
+// step k+1 when it lies in this cycle (gmres.cpp:278-324).
+__global__ void k_givens(Ctl* c, int k, Gate g, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  double* H = c->H;
+  const int ld = c->ld;
+  for (int i = 0; i <= k - 1; ++i) {
+    const double t1 = H[i * ld + k], t2 = H[(i + 1) * ld + k];
+    H[i * ld + k] = c->cs[i] * t1 - c->sn[i] * t2;
+    H[(i + 1) * ld + k] = c->sn[i] * t1 + c->cs[i] * t2;
+  }
+  const double hk = H[k * ld + k], hk1 = H[(k + 1) * ld + k];
+  const double alpha = sqrt(hk * hk + hk1 * hk1);
+  c->cs[k] = hk / alpha;
+  c->sn[k] = -hk1 / alpha;
+  c->e[k + 1] = c->sn[k] * c->e[k];
+  c->e[k] = c->cs[k] * c->e[k];
+  H[k * ld + k] = c->cs[k] * H[k * ld + k] - c->sn[k] * H[(k + 1) * ld + k];
+  H[(k + 1) * ld + k] = 0.0;
+  c->count++;
+  c->steps++;
+  c->tsteps++;
+  if (k + 1 < c->R - 1) {
+    const double err = fabs(c->e[k + 1]) / c->bnorm;
+    ctl_hist(c, (double)(c->count + 1), err);
+    if (c->count + 1 >= c->maxloop) {
+      c->ksz = k + 1;
+      ctl_stop(c, 2);
+    } else if (err <= c->eps) {
+      c->ksz = k + 1;
+      ctl_stop(c, 0);
+    } else if (!finite(err)) {
+      c->ksz = k + 1;
+      ctl_stop(c, 3);
+    }
+  }
+}
+
+// ------------------------------------------------------ k-skip Gram -----
+// CG: sums[0] = (r,r); delta_i = (r,Ar_i) i<2K; eta_i = (r,Ap_i) i<2K+1;
+// zeta_i = (p,Ap_i) i<2K+2.  BiCG (shadow rs, ps): sums[0] = (rs,r),
+// theta_i = (rs,Ar_i) i<2K, eta_i = (rs,Ap_i), rho_i = (ps,Ar_i) i<2K+1,
+// phi_i = (ps,Ap_i) i<2K+2.  Ar/Ap are column-major blocks with stride ld.
+// K is a template parameter so the accumulators stay in registers.
+template <int K, bool BICG>
+struct OpGram {
+  static constexpr int NV = BICG ? 8 * K + 5 : 6 * K + 4;
+  int64_t ld;
+  const double *r, *p, *rs, *ps, *Ar, *Ap;
+  Ctl* c;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double (&acc)[NV]) {
+    if constexpr (!BICG) {
+      const double ri = r[i], pi = p[i];
+      acc[0] += ri * ri;
+#pragma unroll
+      for (int v = 0; v < 2 * K; ++v) acc[1 + v] += ri * Ar[v * ld + i];
+#pragma unroll
+      for (int v = 0; v < 2 * K + 2; ++v) {
+        const double a = Ap[v * ld + i];
+        if (v < 2 * K + 1) acc[1 + 2 * K + v] += ri * a;
+        acc[1 + 4 * K + 1 + v] += pi * a;
+      }
+    } else {
+      const double rsi = rs[i], psi = ps[i];
+      acc[0] += rsi * r[i];
+#pragma unroll
+      for (int v = 0; v < 2 * K + 1; ++v) {
+        const double a = Ar[v * ld + i];
+        if (v < 2 * K) acc[1 + v] += rsi * a;
+        acc[1 + 2 * K + (2 * K + 1) + v] += psi * a;
+      }
+#pragma unroll
+      for (int v = 0; v < 2 * K + 2; ++v) {
+        const double a = Ap[v * ld + i];
+        if (v < 2 * K + 1) acc[1 + 2 * K + v] += rsi * a;
+        acc[1 + 2 * K + 2 * (2 * K + 1) + v] += psi * a;
+      }
+    }
+  }
+  __device__ void fin(const double* s) {
+    c->gamma = s[0];
+    int o = 1;
+    for (int v = 0; v < 2 * K; ++v) c->delta[v] = s[o + v];  // theta for BiCG
+    o += 2 * K;
+    for (int v = 0; v < 2 * K + 1; ++v) c->eta[v] = s[o + v];
+    o += 2 * K + 1;
+    if constexpr (BICG) {
+      for (int v = 0; v < 2 * K + 1; ++v) c->rho[v] = s[o + v];
+      o += 2 * K + 1;
+      for (int v = 0; v < 2 * K + 2; ++v) c->phi[v] = s[o + v];
+    } else {
+      for (int v = 0; v < 2 * K + 2; ++v) c->zeta[v] = s[o + v];
+    }
+  }
+};
+
+// ------------------------------------------------------- transpose ------
+__global__ void k_col_count(int64_t nnz, const int32_t* __restrict__ col, int32_t* cnt) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz; j += stride)
+    atomicAdd(cnt + col[j], 1);
+}
+
+// Exclusive scan, phase 1: per-block sums of `in` (block of 1024 elements).
+__global__ void k_scan_blocks(int64_t n, const int32_t* __restrict__ in, int64_t* bsum) {
+  __shared__ int64_t s[kBlock];
+  const int64_t base = (int64_t)blockIdx.x * 1024;
+  int64_t t = 0;
+  for (int q = 0; q < 4; ++q) {
+    int64_t i = base + threadIdx.x * 4 + q;
+    if (i < n) t += in[i];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+// Phase 2: serial exclusive scan of the block sums (few thousand values).
+__global__ void k_scan_serial(int64_t nb, int64_t* bsum) {
+  int64_t acc = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    int64_t v = bsum[b];
+    bsum[b] = acc;
+    acc += v;
+  }
+  bsum[nb] = acc;
+}
+// Phase 3: out[i] = offset + exclusive prefix within the block; out[n] = total.
+__global__ void k_scan_apply(int64_t n, const int32_t* __restrict__ in, const int64_t* bsum,
+                             int32_t* out) {
+  __shared__ int64_t s[kBlock];
+  const int64_t base = (int64_t)blockIdx.x * 1024;
+  int64_t v[4];
+  int64_t t = 0;
+  for (int q = 0; q < 4; ++q) {
+    int64_t i = base + threadIdx.x * 4 + q;
+    v[q] = i < n ? in[i] : 0;
+    t += v[q];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over the 256 thread totals
+  for (int o = 1; o < kBlock; o <<= 1) {
+    int64_t a = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int64_t run = bsum[blockIdx.x] + s[threadIdx.x] - t;
+  for (int q = 0; q < 4; ++q) {
+    int64_t i = base + threadIdx.x * 4 + q;
+    if (i < n) out[i] = (int32_t)run;
+    run += v[q];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = (int32_t)bsum[gridDim.x];
+}
+
+// Scatter entries of A into column buckets of A^T (order inside a bucket is
+// fixed afterwards by k_sort_segments).
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_tscatter(int64_t n, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ tptr, int32_t* fill,
+                                                     int32_t* tcol, int32_t* tperm) {
+  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / L);
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / L) + grp; row < n; row += stride) {
+    for (int j = ptr[row] + lane; j < ptr[row + 1]; j += L) {
+      const int c = col[j];
+      const int d = tptr[c] + atomicAdd(fill + c, 1);
+      tcol[d] = (int32_t)row;
+      tperm[d] = j;
+    }
+  }
+}
+
+// Sort each A^T row segment by column (= row of A), carrying the value
+// permutation: one thread per segment, insertion sort (segments <= 64).
+__global__ void k_sort_small(int64_t n, const int32_t* __restrict__ tptr, int32_t* tcol, int32_t* tperm,
+                             int32_t* long_rows, int32_t* n_long) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    const int s = tptr[r], e = tptr[r + 1];
+    if (e - s > 64) {
+      long_rows[atomicAdd(n_long, 1)] = (int32_t)r;
+      continue;
+    }
+    for (int a = s + 1; a < e; ++a) {
+      const int32_t kc = tcol[a];
+      const int32_t kp = tperm[a];
+      int b = a - 1;
+      while (b >= s && tcol[b] > kc) {
+        tcol[b + 1] = tcol[b];
+        tperm[b + 1] = tperm[b];
+        --b;
+      }
+      tcol[b + 1] = kc;
+      tperm[b + 1] = kp;
+    }
+  }
+}
+
+// One workgroup per long segment (<= kSortMax entries): bitonic sort in LDS.
+constexpr int kSortMax = 8192;
+__global__ __launch_bounds__(1024) void k_sort_long(const int32_t* __restrict__ tptr,
+                                                    const int32_t* __restrict__ long_rows, int32_t* tcol,
+                                                    int32_t* tperm) {
+  __shared__ int32_t kc[kSortMax];
+  __shared__ int32_t kp[kSortMax];
+  const int32_t r = long_rows[blockIdx.x];
+  const int s = tptr[r], e = tptr[r + 1], len = e - s;
+  int P = 1;
+  while (P < len) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    kc[i] = i < len ? tcol[s + i] : 0x7fffffff;
+    kp[i] = i < len ? tperm[s + i] : 0;
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          if ((kc[i] > kc[j]) == up) {
+            int32_t a = kc[i]; kc[i] = kc[j]; kc[j] = a;
+            int32_t b = kp[i]; kp[i] = kp[j]; kp[j] = b;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    tcol[s + i] = kc[i];
+    tperm[s + i] = kp[i];
+  }
+}
+
+template <class Tin, class Tout>
+__global__ void k_gather(int64_t m, const int32_t* __restrict__ perm, const Tin* __restrict__ in,
+                         Tout* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride)
+    out[j] = (Tout)in[perm[j]];
+}
+
+template <class Tin, class Tout>
+__global__ void k_convert(int64_t m, const Tin* __restrict__ in, Tout* __restrict__ out, Gate g) {
+  if (g.off()) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride)
+    out[j] = (Tout)in[j];
+}
+
+}  // namespace ks

@@ -1,0 +1,817 @@
+// ks_levels.h - device-resident Krylov solvers.
+//
+// A Level is one solve (outer, or the inner solve of a variable-
+// preconditioned outer step).  It enqueues a static schedule of gated
+// kernels: enqueue_init() once, then enqueue_unit(u) for u < max_units()
+// (one iteration, one Arnoldi step or one k-skip block), then
+// enqueue_finish().  Stop tests run on the device; once a level has
+// stopped its remaining kernels return at once.  Outer levels are driven by
+// run_outer(), which enqueues batches of units and checks the host-mapped
+// stop flag with one batch of look-ahead; inner levels are enqueued whole,
+// gated on the outer step's stop flag as well.
+#pragma once
+#include <memory>
+
+#include "ks_dev.h"
+
+namespace ks {
+
+// ------------------------------------------------------- stop tests -----
+__device__ inline void ctl_reset(Ctl* c) {
+  c->stop = 0; c->done = 0; c->status = 2; c->ksz = 0;
+  c->count = 0; c->nhist = 0; c->steps = 0;
+}
+// relres = num / bnorm (0 when both are 0: b = 0 has the solution x = 0)
+__device__ inline double relres(const Ctl* c, double num) {
+  return (c->bnorm == 0.0 && num == 0.0) ? 0.0 : num / c->bnorm;
+}
+__device__ inline bool eps_test(Ctl* c, double idx, double num) {
+  const double err = relres(c, num);
+  ctl_hist(c, idx, err);
+  if (err <= c->eps) { ctl_stop(c, 0); return true; }
+  if (!finite(err)) { ctl_stop(c, 3); return true; }
+  return false;
+}
+// CG family: test at the top of loop `count` (cg.cpp:184-198); the loop
+// ends without a test once count > maxloop.
+__device__ inline void cg_check(Ctl* c, double rr) {
+  if (c->count > c->maxloop) { ctl_stop(c, 2); return; }
+  eps_test(c, (double)c->count, sqrt(rr));
+}
+// k-skip family: test at the top of outer step nloop (kskipcg.cpp:214-227).
+__device__ inline void kskip_check(Ctl* c, double rr) {
+  if (c->count >= c->maxloop) { ctl_stop(c, 2); return; }
+  eps_test(c, (double)(c->count + 1), sqrt(rr));
+}
+// GMRES family: test of step k at iteration count (gmres.cpp:278-324).
+__device__ inline void gm_check(Ctl* c, int k, double num) {
+  const double err = relres(c, num);
+  ctl_hist(c, (double)(c->count + 1), err);
+  if (c->count + 1 >= c->maxloop) { c->ksz = k; ctl_stop(c, 2); }
+  else if (err <= c->eps) { c->ksz = k; ctl_stop(c, 0); }
+  else if (!finite(err)) { c->ksz = k; ctl_stop(c, 3); }
+}
+
+// ------------------------------------------------------------- ops ------
+// x = 0 (optional), r = (T)src, p = r (optional), acc (r,r); sets the loop.
+// MODE 0: CG-like (count=1, test), 1: k-skip (count=0, test), 2: no test.
+template <class T, int MODE>
+struct OpCgInit {
+  const double* src;
+  T *x, *r, *p, *rs, *ps;
+  Ctl* c;
+  int zero_x, set_bnorm;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) {
+    const T ri = (T)src[i];
+    r[i] = ri;
+    if (p) p[i] = ri;
+    if (rs) rs[i] = ri;
+    if (ps) ps[i] = ri;
+    if (zero_x) x[i] = (T)0;
+    acc[0] += (double)ri * (double)ri;
+  }
+  __device__ void fin(const double* s) {
+    ctl_reset(c);
+    if (set_bnorm) c->bnorm = sqrt(s[0]);
+    c->rr = s[0];
+    if (MODE == 0) { c->count = 1; cg_check(c, s[0]); }
+    else if (MODE == 1) { c->count = 0; kskip_check(c, s[0]); }
+  }
+};
+
+struct FinAlpha {
+  Ctl* c;
+  __device__ void operator()(const double* s) const { c->dotv = s[0]; c->alpha = c->rr / s[0]; }
+};
+
+// x += a p; r -= a q; acc (r,r).  MODE 0: CG (beta, rr, count++ and test),
+// 1: VPCG (count++ and test only).
+template <class T, int MODE>
+struct OpCgUpdate {
+  T *x, *r;
+  const T *p, *q;
+  Ctl* c;
+  double a;
+  __device__ void prep() { a = c->alpha; }
+  __device__ void elem(int64_t i, double* acc) {
+    x[i] = (T)(a * (double)p[i] + (double)x[i]);
+    const T ri = (T)((-a) * (double)q[i] + (double)r[i]);
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+  }
+  __device__ void fin(const double* s) {
+    if (MODE == 0) { c->beta = s[0] / c->rr; c->rr = s[0]; }
+    c->count++;
+    c->steps++;
+    c->tsteps++;
+    cg_check(c, s[0]);
+  }
+};
+
+// p = beta p + r  (blas::Scalar_axy(beta, p, r, p)), optionally twice.
+template <class T>
+struct OpXpby {
+  T* p;
+  const T* r;
+  T* p2;
+  const T* r2;
+  const Ctl* c;
+  double b;
+  __device__ void prep() { b = c->beta; }
+  __device__ void elem(int64_t i, double*) {
+    p[i] = (T)(b * (double)p[i] + (double)r[i]);
+    if (p2) p2[i] = (T)(b * (double)p2[i] + (double)r2[i]);
+  }
+  __device__ void fin(const double*) {}
+};
+
+// x += a p; r -= a q   (NV 0), or with acc (r,r) and the k-skip test (NV 1).
+template <class T, int NV, int MODE>  // MODE 0: BiCG axpy2, 1: k-skip BiCG last
+struct OpAxpy2 {
+  T *x, *r;
+  const T *p, *q;
+  Ctl* c;
+  int kp1;
+  double a;
+  __device__ void prep() { a = c->alpha; }
+  __device__ void elem(int64_t i, double* acc) {
+    x[i] = (T)(a * (double)p[i] + (double)x[i]);
+    const T ri = (T)((-a) * (double)q[i] + (double)r[i]);
+    r[i] = ri;
+    if (NV) acc[0] += (double)ri * (double)ri;
+  }
+  __device__ void fin(const double* s) {
+    if (NV) { c->count += kp1; c->steps += kp1; c->tsteps += kp1; kskip_check(c, s[0]); }
+  }
+};
+
+// BiCG: rs -= a q2; acc0 (rs,r), acc1 (r,r); beta, rr, count++, test.
+template <class T>
+struct OpBicgUpdate {
+  T* rs;
+  const T *q2, *r;
+  Ctl* c;
+  double a;
+  __device__ void prep() { a = c->alpha; }
+  __device__ void elem(int64_t i, double* acc) {
+    const T v = (T)((-a) * (double)q2[i] + (double)rs[i]);
+    rs[i] = v;
+    const double ri = (double)r[i];
+    acc[0] += (double)v * ri;
+    acc[1] += ri * ri;
+  }
+  __device__ void fin(const double* s) {
+    c->beta = s[0] / c->rr;
+    c->rr = s[0];
+    c->count++;
+    c->steps++;
+    c->tsteps++;
+    cg_check(c, s[1]);
+  }
+};
+
+// k-skip: x += a p; r -= a Av; p = b p + r; last step of a block adds the
+// (r,r) test (NV 1).
+template <class T, int NV>
+struct OpAxpy3 {
+  T *x, *r, *p;
+  const T* q;
+  Ctl* c;
+  int kp1;
+  double a, b;
+  __device__ void prep() { a = c->alpha; b = c->beta; }
+  __device__ void elem(int64_t i, double* acc) {
+    const double pi = (double)p[i];
+    x[i] = (T)(a * pi + (double)x[i]);
+    const T ri = (T)((-a) * (double)q[i] + (double)r[i]);
+    r[i] = ri;
+    p[i] = (T)(b * pi + (double)ri);
+    if (NV) acc[0] += (double)ri * (double)ri;
+  }
+  __device__ void fin(const double* s) {
+    if (NV) { c->count += kp1; c->steps += kp1; c->tsteps += kp1; kskip_check(c, s[0]); }
+  }
+};
+
+// VPCG: p = z, acc0 (r,z), acc1 (r,r); test at loop 1.
+struct OpVpInit {
+  double* p;
+  const double *z, *r;
+  Ctl* c;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) {
+    const double zi = z[i], ri = r[i];
+    p[i] = zi;
+    acc[0] += ri * zi;
+    acc[1] += ri * ri;
+  }
+  __device__ void fin(const double* s) {
+    c->rr = s[0];
+    c->count = 1;
+    cg_check(c, s[1]);
+  }
+};
+// VPCG beta: (r',z')/(r,z) (vpcg.cpp:753) or flexible -alpha (z',Ap)/(r,z).
+struct OpVpBeta {
+  const double *r, *z, *q;
+  Ctl* c;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) {
+    const double zi = z[i];
+    acc[0] += r[i] * zi;
+    acc[1] += zi * q[i];
+  }
+  __device__ void fin(const double* s) {
+    c->beta = (c->compat & KS_COMPAT_VPCG_FR) ? s[0] / c->rr : -c->alpha * s[1] / c->rr;
+    c->rr = s[0];
+  }
+};
+
+// out = in / vscale (blas::Scalar_x_div_a)
+template <class T>
+struct OpScale {
+  const T* in;
+  T* out;
+  const Ctl* c;
+  double vs;
+  __device__ void prep() { vs = c->vscale; }
+  __device__ void elem(int64_t i, double*) { out[i] = (T)((double)in[i] / vs); }
+  __device__ void fin(const double*) {}
+};
+
+// GMRES cycle start: r = (T)src (first cycle, x0 = 0) and the norm.
+template <class T>
+struct OpGmFirst {
+  const double* src;
+  T *r, *x;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) {
+    const T ri = (T)src[i];
+    r[i] = ri;
+    if (x) x[i] = (T)0;
+    acc[0] += (double)ri * (double)ri;
+  }
+};
+struct FinGmCycle {
+  Ctl* c;
+  int first, set_bnorm;
+  __device__ void operator()(const double* s) const {
+    const double t = sqrt(s[0]);
+    if (first) ctl_reset(c);
+    if (set_bnorm) c->bnorm = t;
+    for (int i = 0; i < c->R; ++i) c->e[i] = 0.0;
+    c->e[0] = t;
+    c->vscale = t;
+    gm_check(c, 0, t);
+  }
+};
+template <class T>
+struct OpGmFirstFin : OpGmFirst<T> {
+  FinGmCycle f;
+  __device__ void fin(const double* s) { f(s); }
+};
+struct FinGmNorm {
+  Ctl* c;
+  int k;
+  __device__ void operator()(const double* s) const {
+    const double h = sqrt(s[0]);
+    c->H[(k + 1) * c->ld + k] = h;
+    c->vscale = h;
+  }
+};
+
+// Residual/bnorm helper ops for the API entry points.
+struct OpNorm2 {
+  const double* x;
+  double* out;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) { acc[0] += x[i] * x[i]; }
+  __device__ void fin(const double* s) { *out = sqrt(s[0]); }
+};
+struct FinSetBnorm {
+  Ctl* c;
+  __device__ void operator()(const double* s) const { c->bnorm = sqrt(s[0]); }
+};
+struct OpSetBnorm {
+  const double* b;
+  Ctl* c;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) { acc[0] += b[i] * b[i]; }
+  __device__ void fin(const double* s) { c->bnorm = sqrt(s[0]); }
+};
+
+// k-skip CG scalar recurrences for inner step s (kskipcg.cpp:266-288).
+struct OpKskipScalar {
+  Ctl* c;
+  int s;
+  __device__ void run() {
+    const int k = c->kskip;
+    double* delta = c->delta;
+    double* eta = c->eta;
+    double* zeta = c->zeta;
+    const double alpha = c->gamma / zeta[0];
+    const double beta = alpha * zeta[1] / zeta[0] - 1.0;
+    if (c->fix == 1) {
+      c->gamma = beta * c->gamma;
+    } else if (c->fix == 2) {
+      const double t0 = c->gamma - alpha * eta[0];
+      const double t1 = eta[0] - alpha * zeta[1];
+      c->gamma = t0 - alpha * t1;
+    }
+    const int jm = 2 * k - 2 * s;
+    if (c->compat & KS_COMPAT_KSKIP_AS_WRITTEN) {
+      for (int j = 0; j < jm; ++j) {
+        delta[j] = delta[j] - 2 * alpha * eta[j + 1] + alpha * alpha * eta[j + 2];
+        const double eta_old = eta[j];
+        eta[j] = delta[j] + beta * zeta[j + 1] - alpha * beta * zeta[j + 1];
+        zeta[j] = eta[j + 1] + beta * eta_old + beta * beta * zeta[j] - alpha * beta * zeta[j + 1];
+      }
+    } else {
+      for (int j = 0; j < jm; ++j) {
+        const double dn = delta[j] - 2 * alpha * eta[j + 1] + alpha * alpha * zeta[j + 2];
+        const double en = dn + beta * eta[j] - alpha * beta * zeta[j + 1];
+        const double zn = en + beta * eta[j] - alpha * beta * zeta[j + 1] + beta * beta * zeta[j];
+        delta[j] = dn;
+        eta[j] = en;
+        zeta[j] = zn;
+      }
+    }
+    c->alpha = alpha;
+    c->beta = beta;
+  }
+};
+// k-skip BiCG scalar recurrences (kskipbicg.cpp:298-314).
+struct OpKskipBicgScalar {
+  Ctl* c;
+  int s;
+  __device__ void run() {
+    const int k = c->kskip;
+    double* th = c->delta;
+    double* eta = c->eta;
+    double* rho = c->rho;
+    double* phi = c->phi;
+    const double alpha = c->gamma / phi[0];
+    const double beta = 1.0 - (eta[0] + rho[0] - alpha * phi[1]) / phi[0];
+    c->gamma = c->gamma * beta;
+    const int jm = 2 * k - 2 * s;
+    for (int j = 0; j < jm; ++j) {
+      th[j] = th[j] - alpha * eta[j + 1] - alpha * rho[j + 1] + alpha * alpha * phi[j + 2];
+      const double t = th[j] - alpha * beta * phi[j + 1];
+      eta[j] = t + beta * eta[j];
+      rho[j] = t + beta * rho[j];
+      phi[j] = eta[j] + rho[j] - th[j] + beta * beta * phi[j];
+    }
+    c->alpha = alpha;
+    c->beta = beta;
+  }
+};
+
+// ----------------------------------------------------------- Level -----
+struct LevelCfg {
+  int maxloop = 0;
+  double eps = 0;
+  int restart = 0;
+  int kskip = 0;
+  int fix = 0;
+  unsigned compat = 0;
+  int64_t hist_cap = 0;
+};
+
+struct Level {
+  ks_ctx* c = nullptr;
+  int64_t n = 0;
+  LevelCfg cfg;
+  DBuf<Ctl> dctl;
+  Ctl hctl{};
+  DBuf<double> arr;   // H, e, cs, sn, y, delta, eta, zeta, rho, phi
+  DBuf<double> hist;
+  const int* outer_stop = nullptr;  // set for inner levels
+  bool x0given = false;
+  virtual ~Level() {}
+
+  Ctl* ctl() const { return dctl.p; }
+  const int* stop_flag() const { return &dctl.p->stop; }
+  Gate gstep() const { return Gate{outer_stop, &dctl.p->stop}; }
+  Gate gdone() const { return Gate{outer_stop, &dctl.p->done}; }
+  Gate gouter() const { return Gate{outer_stop, nullptr}; }
+
+  void setup_ctl(int R, int k) {
+    hctl = Ctl{};
+    hctl.status = 2;
+    hctl.maxloop = cfg.maxloop;
+    hctl.eps = cfg.eps;
+    hctl.R = R;
+    hctl.ld = R > 0 ? R : 1;
+    hctl.kskip = k;
+    hctl.fix = cfg.fix;
+    hctl.compat = cfg.compat;
+    const int64_t nH = R > 0 ? (int64_t)(R + 1) * R : 0;
+    const int64_t nsmall = R > 0 ? 4 * (int64_t)R : 0;
+    const int64_t nk = 5 * (int64_t)(2 * k + 3);
+    arr.alloc((size_t)(nH + nsmall + nk + 8));
+    KS_HIPCK(hipMemsetAsync(arr.p, 0, arr.n * sizeof(double), c->stream));
+    double* q = arr.p;
+    if (R > 0) {
+      hctl.H = q; q += nH;
+      hctl.e = q; q += R;
+      hctl.cs = q; q += R;
+      hctl.sn = q; q += R;
+      hctl.y = q; q += R;
+    }
+    hctl.delta = q; q += 2 * k + 3;
+    hctl.eta = q; q += 2 * k + 3;
+    hctl.zeta = q; q += 2 * k + 3;
+    hctl.rho = q; q += 2 * k + 3;
+    hctl.phi = q; q += 2 * k + 3;
+    if (cfg.hist_cap > 0) {
+      hist.alloc((size_t)(2 * cfg.hist_cap));
+      hctl.hist = hist.p;
+      hctl.hist_cap = cfg.hist_cap;
+    }
+    dctl.alloc(1);
+    upload_ctl();
+  }
+  void upload_ctl() { KS_HIPCK(hipMemcpyAsync(dctl.p, &hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream)); }
+
+  // b (FP64, device) is the right-hand side; x_out (FP64, device) receives
+  // the solution.  Inner levels always start from x = 0.
+  virtual void enqueue_init(const double* b, double* x_out) = 0;
+  virtual void enqueue_unit(int64_t u) = 0;
+  virtual int64_t max_units() const = 0;
+  virtual void enqueue_finish() {}
+  virtual int64_t reported_iters(const Ctl& h) const { return h.count; }
+  virtual double unit_seconds_estimate() const = 0;
+};
+
+// ------------------------------------------------------------- CG ------
+template <class T>
+struct CgLevel : Level {
+  MatView<T> A;
+  DBuf<T> r, p, q, xown;
+  T* x = nullptr;
+  double* xout = nullptr;
+  DBuf<double> r64;  // outer with x0: residual source
+  void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf) {
+    c = cx; A = a; n = a.n; cfg = cf;
+    r.alloc(n); p.alloc(n); q.alloc(n);
+    if (sizeof(T) != 8) xown.alloc(n);
+    setup_ctl(0, 0);
+  }
+  void enqueue_init(const double* b, double* x_out) override {
+    xout = x_out;
+    x = sizeof(T) == 8 ? (T*)x_out : xown.p;
+    const double* src = b;
+    if (x0given) {  // outer FP64 only: bnorm, then r = b - A x0
+      launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinNone>{b, r.p, FinNone{}}, gouter(), 8.0 * n);
+      src = (const double*)r.p;
+    }
+    OpCgInit<T, 0> op{src, x, r.p, p.p, nullptr, nullptr, ctl(), !x0given, !x0given};
+    launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (x0given ? 24.0 : 32.0) * n);
+  }
+  void enqueue_unit(int64_t) override {
+    launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{p.p, q.p, FinAlpha{ctl()}}, gstep(), sizeof(T) * (double)n);
+    launch_map<1>(c, n, OpCgUpdate<T, 0>{x, r.p, p.p, q.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+    launch_map<0>(c, n, OpXpby<T>{p.p, r.p, nullptr, nullptr, ctl(), 0.0}, gstep(), KS_K_AXPY, 3.0 * sizeof(T) * n);
+  }
+  int64_t max_units() const override { return cfg.maxloop > 0 ? cfg.maxloop : 0; }
+  void enqueue_finish() override {
+    if (sizeof(T) != 8) {
+      k_convert<T, double><<<grid_map(n), kBlock, 0, c->stream>>>(n, x, xout, gouter());
+      KS_HIPCK(hipGetLastError());
+    }
+  }
+  double unit_seconds_estimate() const override {
+    return std::max(3 * 4e-6, (spmv_bytes<T>(n, A.nnz) + 10.0 * sizeof(T) * n) / 4e12);
+  }
+};
+
+// ----------------------------------------------------------- BiCG ------
+template <class T>
+struct BicgLevel : Level {
+  MatView<T> A, AT;
+  DBuf<T> r, rs, p, ps, q, xown;
+  T* x = nullptr;
+  double* xout = nullptr;
+  void setup(ks_ctx* cx, const MatView<T>& a, const MatView<T>& at, const LevelCfg& cf) {
+    c = cx; A = a; AT = at; n = a.n; cfg = cf;
+    r.alloc(n); rs.alloc(n); p.alloc(n); ps.alloc(n); q.alloc(n);
+    if (sizeof(T) != 8) xown.alloc(n);
+    setup_ctl(0, 0);
+  }
+  void enqueue_init(const double* b, double* x_out) override {
+    xout = x_out;
+    x = sizeof(T) == 8 ? (T*)x_out : xown.p;
+    const double* src = b;
+    if (x0given) {
+      launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinNone>{b, r.p, FinNone{}}, gouter(), 8.0 * n);
+      src = (const double*)r.p;
+    }
+    OpCgInit<T, 0> op{src, x, r.p, p.p, rs.p, ps.p, ctl(), !x0given, !x0given};
+    launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, 48.0 * n);
+  }
+  void enqueue_unit(int64_t) override {
+    launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{ps.p, q.p, FinAlpha{ctl()}}, gstep(), sizeof(T) * (double)n);
+    launch_map<0>(c, n, OpAxpy2<T, 0, 0>{x, r.p, p.p, q.p, ctl(), 0, 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+    launch_spmv<T, 0>(c, AT, ps.p, EpiStore<T>{q.p}, gstep());
+    launch_map<2>(c, n, OpBicgUpdate<T>{rs.p, q.p, r.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 4.0 * sizeof(T) * n);
+    launch_map<0>(c, n, OpXpby<T>{p.p, r.p, ps.p, rs.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+  }
+  int64_t max_units() const override { return cfg.maxloop > 0 ? cfg.maxloop : 0; }
+  void enqueue_finish() override {
+    if (sizeof(T) != 8) {
+      k_convert<T, double><<<grid_map(n), kBlock, 0, c->stream>>>(n, x, xout, gouter());
+      KS_HIPCK(hipGetLastError());
+    }
+  }
+  double unit_seconds_estimate() const override {
+    return std::max(5 * 4e-6, (2 * spmv_bytes<T>(n, A.nnz) + 20.0 * sizeof(T) * n) / 4e12);
+  }
+};
+
+// --------------------------------------------------- projection helper --
+// h[i] = (w, V_i), i <= k, in launches of up to 32 columns.
+template <class T>
+void launch_proj(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T* w, double* hcol, int ldh,
+                 Gate g) {
+  for (int i0 = 0; i0 < nv; i0 += 32) {
+    const int m = std::min(32, nv - i0);
+    const int G = grid_map(n);
+    const double bytes = (double)sizeof(T) * n * (m + 1);
+    const T* Vi = V + (int64_t)i0 * ldv;
+    double* hi = hcol + (int64_t)i0 * ldh;
+    c->prof.begin(c->stream, KS_K_CGS);
+    if (m <= 4) k_proj<T, 4><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
+    else if (m <= 8) k_proj<T, 8><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
+    else if (m <= 16) k_proj<T, 16><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
+    else k_proj<T, 32><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
+    c->prof.end(c->stream);
+    KS_HIPCK(hipGetLastError());
+  }
+}
+
+template <class T, class Fin>
+void launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, T* w, const double* hcol, int ldh,
+                       Fin f, Gate g) {
+  const double bytes = (double)sizeof(T) * n * (nv + 2);
+  c->prof.begin(c->stream, KS_K_CGS);
+  k_cgs_update<T, Fin><<<grid_map(n), kBlock, sizeof(double) * std::max(nv, 1), c->stream>>>(
+      n, V, ldv, nv, w, hcol, ldh, f, g, red(c), stat(c, KS_K_CGS, bytes));
+  c->prof.end(c->stream);
+  KS_HIPCK(hipGetLastError());
+}
+
+inline int64_t pad64(int64_t n) { return (n + 63) & ~(int64_t)63; }
+
+// ---------------------------------------------------------- GMRES ------
+// gmres::solve (gmres.cpp:232-423); with `inner` set it is vpgmres::solve
+// (vpgmres.cpp:746-944): z_k = M(v_k) into Z[k], w = A z_k, x from Z.
+template <class T>
+struct GmresLevel : Level {
+  MatView<T> A;
+  int R = 0;
+  int64_t ldv = 0;
+  DBuf<T> V, w, r, xown;
+  DBuf<double> Z;  // VP outer only (T = double)
+  T* x = nullptr;
+  double* xout = nullptr;
+  const double* b = nullptr;
+  std::unique_ptr<Level> inner;
+  void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf, std::unique_ptr<Level> in = nullptr) {
+    c = cx; A = a; n = a.n; cfg = cf; R = cf.restart;
+    KS_CHECK(R >= 2, KS_ERR_BAD_INPUT, "GMRES restart (-R/-r) must be >= 2");
+    ldv = pad64(n);
+    V.alloc((size_t)ldv * R);
+    w.alloc(n); r.alloc(n);
+    if (sizeof(T) != 8) xown.alloc(n);
+    setup_ctl(R, 0);
+    inner = std::move(in);
+    if (inner) {
+      Z.alloc((size_t)ldv * R);
+      inner->outer_stop = stop_flag_ptr();
+    }
+  }
+  const int* stop_flag_ptr() { return &dctl.p->stop; }
+  int64_t cycles() const {
+    if (cfg.maxloop <= 0) return 0;
+    return (int64_t)(cfg.maxloop - 1) / (R - 1) + 1;
+  }
+  int64_t max_units() const override { return cycles() * (R - 1); }
+  void enqueue_init(const double* bb, double* x_out) override {
+    b = bb;
+    xout = x_out;
+    x = sizeof(T) == 8 ? (T*)x_out : xown.p;
+    if (x0given) launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+  }
+  void cycle_start(int64_t cyc) {
+    if (cyc == 0 && !x0given) {
+      OpGmFirstFin<T> op;
+      op.src = b; op.r = r.p; op.x = x; op.f = FinGmCycle{ctl(), 1, 1};
+      launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (8.0 + 2.0 * sizeof(T)) * n);
+    } else {
+      Gate g = cyc == 0 ? gouter() : gstep();
+      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinGmCycle>{b, r.p, FinGmCycle{ctl(), cyc == 0 ? 1 : 0, 0}}, g, 8.0 * n);
+    }
+    launch_map<0>(c, n, OpScale<T>{r.p, V.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
+  }
+  void enqueue_unit(int64_t u) override {
+    const int64_t cyc = u / (R - 1);
+    const int k = (int)(u % (R - 1));
+    if (k == 0) cycle_start(cyc);
+    T* vk = V.p + (int64_t)k * ldv;
+    if (inner) {
+      double* zk = Z.p + (int64_t)k * ldv;
+      inner->enqueue_init((const double*)vk, zk);
+      for (int64_t iu = 0; iu < inner->max_units(); ++iu) inner->enqueue_unit(iu);
+      inner->enqueue_finish();
+      launch_spmv<T, 0>(c, A, (const T*)zk, EpiStore<T>{w.p}, gstep());
+    } else {
+      launch_spmv<T, 0>(c, A, vk, EpiStore<T>{w.p}, gstep());
+    }
+    Ctl* h = ctl();
+    launch_proj<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, gstep());
+    launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
+    launch_map<0>(c, n, OpScale<T>{w.p, V.p + (int64_t)(k + 1) * ldv, h, 0.0}, gstep(), KS_K_AXPY,
+                  2.0 * sizeof(T) * n);
+    k_givens<<<1, 1, 0, c->stream>>>(h, k, gstep(), stat(c, KS_K_SCALAR, 0));
+    KS_HIPCK(hipGetLastError());
+    if (k == R - 2) cycle_end();
+  }
+  void cycle_end() {
+    Ctl* h = ctl();
+    k_hye<<<1, 1, 0, c->stream>>>(h, gdone());
+    KS_HIPCK(hipGetLastError());
+    const int last = (cfg.compat & KS_COMPAT_LAST_TERM) ? 1 : 0;
+    const double bytes = (double)sizeof(T) * n * 2 + (inner ? 8.0 : (double)sizeof(T)) * n * (R - 1);
+    c->prof.begin(c->stream, KS_K_CGS);
+    if (inner)
+      k_assemble<T, double><<<grid_map(n), kBlock, sizeof(double) * R, c->stream>>>(
+          n, Z.p, ldv, x, h, last, gdone(), stat(c, KS_K_CGS, bytes));
+    else
+      k_assemble<T, T><<<grid_map(n), kBlock, sizeof(double) * R, c->stream>>>(
+          n, V.p, ldv, x, h, last, gdone(), stat(c, KS_K_CGS, bytes));
+    c->prof.end(c->stream);
+    KS_HIPCK(hipGetLastError());
+    k_mark_done<<<1, 1, 0, c->stream>>>(h, gdone());
+    KS_HIPCK(hipGetLastError());
+  }
+  void enqueue_finish() override {
+    cycle_end();  // no-op unless the solve stopped in a cycle not yet assembled
+    if (sizeof(T) != 8) {
+      k_convert<T, double><<<grid_map(n), kBlock, 0, c->stream>>>(n, x, xout, gouter());
+      KS_HIPCK(hipGetLastError());
+    }
+  }
+  int64_t reported_iters(const Ctl& hc) const override { return hc.count + 1; }
+  double unit_seconds_estimate() const override {
+    double s = (spmv_bytes<T>(n, A.nnz) + (double)sizeof(T) * n * (R + 8)) / 4e12 + 5 * 4e-6;
+    if (inner) s += inner->max_units() * inner->unit_seconds_estimate();
+    return s;
+  }
+};
+
+// ------------------------------------------------------- k-skip CG -----
+template <class T, bool BICG>
+struct KskipLevel : Level {
+  MatView<T> A, AT;
+  int K = 0;
+  int64_t ldv = 0;
+  DBuf<double> r, p, rs, ps, q, Ar, Ap;
+  double* x = nullptr;
+  void setup(ks_ctx* cx, const MatView<T>& a, const MatView<T>& at, const LevelCfg& cf) {
+    static_assert(sizeof(T) == 8, "k-skip levels run in FP64");
+    c = cx; A = a; AT = at; n = a.n; cfg = cf; K = cf.kskip;
+    KS_CHECK(K >= 1 && K <= 7, KS_ERR_UNSUPPORTED, "k-skip depth must be in [1, 7]");
+    ldv = pad64(n);
+    r.alloc(n); p.alloc(n); q.alloc(n);
+    if (BICG) { rs.alloc(n); ps.alloc(n); }
+    Ar.alloc((size_t)ldv * (2 * K + 1));
+    Ap.alloc((size_t)ldv * (2 * K + 2));
+    setup_ctl(0, K);
+  }
+  void enqueue_init(const double* b, double* x_out) override {
+    x = x_out;
+    const double* src = b;
+    if (x0given) {
+      launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinNone>{b, r.p, FinNone{}}, gouter(), 8.0 * n);
+      src = r.p;
+    }
+    OpCgInit<T, 1> op{src, x, r.p, p.p, BICG ? rs.p : nullptr, BICG ? ps.p : nullptr, ctl(), !x0given, !x0given};
+    launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (BICG ? 48.0 : 32.0) * n);
+  }
+  int64_t max_units() const override { return cfg.maxloop > 0 ? (cfg.maxloop + K) / (K + 1) : 0; }
+  template <int KK>
+  void gram() {
+    using Op = OpGram<KK, BICG>;
+    Op op{ldv, r.p, p.p, BICG ? rs.p : nullptr, BICG ? ps.p : nullptr, Ar.p, Ap.p, ctl()};
+    const double bytes = 8.0 * n * (BICG ? (4 * KK + 7) : (4 * KK + 5));
+    launch_map<Op::NV>(c, n, op, gstep(), KS_K_BASIS, bytes);
+  }
+  void enqueue_unit(int64_t) override {
+    // matrix-powers basis (cuda::Kskip_cg_bicg_base, cudaFunction.cu:984-1026)
+    const Gate g = gstep();
+    for (int ii = 0; ii < 2 * K + 2; ++ii) {
+      const double* sr = ii == 0 ? r.p : Ar.p + (int64_t)(ii - 1) * ldv;
+      const double* sp = ii == 0 ? p.p : Ap.p + (int64_t)(ii - 1) * ldv;
+      const bool need_r = BICG ? ii < 2 * K + 1 : ii < 2 * K;  // CG never reads Ar[2K]
+      if (need_r)
+        launch_spmv2<T>(c, A, sr, sp, Ar.p + (int64_t)ii * ldv, Ap.p + (int64_t)ii * ldv, g);
+      else
+        launch_spmv<T, 0>(c, A, sp, EpiStore<T>{Ap.p + (int64_t)ii * ldv}, g);
+    }
+    switch (K) {
+      case 1: gram<1>(); break;
+      case 2: gram<2>(); break;
+      case 3: gram<3>(); break;
+      case 4: gram<4>(); break;
+      case 5: gram<5>(); break;
+      case 6: gram<6>(); break;
+      default: gram<7>(); break;
+    }
+    for (int s = 0; s <= K; ++s) {
+      if (BICG) launch_scalar(c, OpKskipBicgScalar{ctl(), s}, g);
+      else launch_scalar(c, OpKskipScalar{ctl(), s}, g);
+      launch_spmv<T, 0>(c, A, p.p, EpiStore<T>{q.p}, g);
+      if (!BICG) {
+        if (s < K)
+          launch_map<0>(c, n, OpAxpy3<T, 0>{x, r.p, p.p, q.p, ctl(), K + 1, 0, 0}, g, KS_K_AXPY, 56.0 * n);
+        else
+          launch_map<1>(c, n, OpAxpy3<T, 1>{x, r.p, p.p, q.p, ctl(), K + 1, 0, 0}, g, KS_K_AXPY, 56.0 * n);
+      } else {
+        if (s < K)
+          launch_map<0>(c, n, OpAxpy2<T, 0, 0>{x, r.p, p.p, q.p, ctl(), K + 1, 0}, g, KS_K_AXPY, 48.0 * n);
+        else
+          launch_map<1>(c, n, OpAxpy2<T, 1, 1>{x, r.p, p.p, q.p, ctl(), K + 1, 0}, g, KS_K_AXPY, 48.0 * n);
+        launch_spmv<T, 0>(c, AT, ps.p, EpiStore<T>{q.p}, g);
+        launch_map<0>(c, n, OpRsUpdate{rs.p, q.p, ctl(), 0}, g, KS_K_AXPY, 24.0 * n);
+        launch_map<0>(c, n, OpXpby<T>{p.p, r.p, ps.p, rs.p, ctl(), 0.0}, g, KS_K_AXPY, 48.0 * n);
+      }
+    }
+  }
+  // k-skip BiCG: rs -= alpha A^T ps
+  struct OpRsUpdate {
+    double* rs;
+    const double* q;
+    const Ctl* c;
+    double a;
+    __device__ void prep() { a = c->alpha; }
+    __device__ void elem(int64_t i, double*) { rs[i] = (-a) * q[i] + rs[i]; }
+    __device__ void fin(const double*) {}
+  };
+  int64_t reported_iters(const Ctl& hc) const override { return BICG ? hc.count + 1 : hc.count - K + 1; }
+  double unit_seconds_estimate() const override {
+    return (double)(5 * K + 8) * 4e-6 + ((4 * K + 3 + K + 1) * spmv_bytes<T>(n, A.nnz)) / 4e12;
+  }
+};
+
+// ------------------------------------------------------------ VPCG -----
+// vpcg::solve (vpcg.cpp:659-760) with z = M(r) from z = 0 each time.
+struct VpcgLevel : Level {
+  MatView<double> A;
+  DBuf<double> r, p, q, z;
+  double* x = nullptr;
+  std::unique_ptr<Level> inner;
+  void setup(ks_ctx* cx, const MatView<double>& a, const LevelCfg& cf, std::unique_ptr<Level> in) {
+    c = cx; A = a; n = a.n; cfg = cf;
+    r.alloc(n); p.alloc(n); q.alloc(n); z.alloc(n);
+    inner = std::move(in);
+    setup_ctl(0, 0);
+    inner->outer_stop = &dctl.p->stop;
+  }
+  void run_inner(const double* v) {
+    inner->enqueue_init(v, z.p);
+    for (int64_t u = 0; u < inner->max_units(); ++u) inner->enqueue_unit(u);
+    inner->enqueue_finish();
+  }
+  void enqueue_init(const double* b, double* x_out) override {
+    x = x_out;
+    const double* src = b;
+    if (x0given) {
+      launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+      launch_spmv<double, 1>(c, A, x, EpiResid<double, FinNone>{b, r.p, FinNone{}}, gouter(), 8.0 * n);
+      src = r.p;
+    }
+    // r = b, x = 0, bnorm; the test of loop 1 comes after z = M(r)
+    OpCgInit<double, 2> op{src, x, r.p, nullptr, nullptr, nullptr, ctl(), !x0given, !x0given};
+    launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, 24.0 * n);
+    run_inner(r.p);
+    launch_map<2>(c, n, OpVpInit{p.p, z.p, r.p, ctl()}, gstep(), KS_K_REDUCE, 24.0 * n);
+  }
+  void enqueue_unit(int64_t) override {
+    launch_spmv<double, 1>(c, A, p.p, EpiDot<double, FinAlpha>{p.p, q.p, FinAlpha{ctl()}}, gstep(), 8.0 * n);
+    launch_map<1>(c, n, OpCgUpdate<double, 1>{x, r.p, p.p, q.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 48.0 * n);
+    run_inner(r.p);
+    launch_map<2>(c, n, OpVpBeta{r.p, z.p, q.p, ctl()}, gstep(), KS_K_REDUCE, 24.0 * n);
+    launch_map<0>(c, n, OpXpby<double>{p.p, z.p, nullptr, nullptr, ctl(), 0.0}, gstep(), KS_K_AXPY, 24.0 * n);
+  }
+  int64_t max_units() const override { return cfg.maxloop > 0 ? cfg.maxloop : 0; }
+  double unit_seconds_estimate() const override {
+    return (spmv_bytes<double>(n, A.nnz) + 150.0 * n) / 4e12 + 5 * 4e-6 +
+           inner->max_units() * inner->unit_seconds_estimate();
+  }
+};
+
+}  // namespace ks

@@ -1,0 +1,159 @@
+"""GPU parity of the BLAS-layer kernels against the CPU oracle, through the C ABI.
+
+Tolerances: a row sum of m terms differs from the oracle's sequential sum by
+at most ~m*eps*sum|a_ij x_j| (different order, FMA), so SpMV is checked
+against 4*m*eps*sum|a x|; reductions against 64*eps*sum|x_i y_i|; the
+transpose (pure integer/permutation work) bit for bit.
+"""
+import numpy as np
+import pytest
+
+from tests import _problems as P
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = 2.0 ** -53
+EPS32 = 2.0 ** -24
+
+
+def _mat(ctx, ptr, col, val, **kw):
+    from solver_cpp_amd import api
+    return api.Matrix(ctx, ptr, col, val, **kw)
+
+
+@pytest.mark.parametrize("case", ["poisson2d", "convdiff3d", "zipf"])
+def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case):
+    if case == "poisson2d":
+        ptr, col, val, _, _ = P.poisson2d(40)
+    elif case == "convdiff3d":
+        ptr, col, val, _, _ = P.convdiff3d(13)
+    else:
+        ptr, col, val, _ = P.zipf_small(5000, 600)
+    n = len(ptr) - 1
+    x = np.random.default_rng(7).standard_normal(n)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    m = np.diff(ptr).max()
+    y = A.spmv(x)
+    yo = oracle.spmv(ptr, col, val, x)
+    bound = 4 * m * EPS64 * P.row_abs_bound(ptr, col, val, x) + 1e-300
+    assert np.all(np.abs(y - yo) <= bound)
+    # transpose: bit-exact against a stable counting sort
+    tp, tc, tv = A.transpose_csr()
+    otp, otc, otv = oracle.transpose(ptr, col, val)
+    assert np.array_equal(tp, otp) and np.array_equal(tc, otc) and np.array_equal(tv, otv)
+    yt = A.spmv(x, trans=True)
+    yto = oracle.spmv(otp, otc, otv, x)
+    assert np.all(np.abs(yt - yto) <= 4 * np.diff(otp).max() * EPS64 * P.row_abs_bound(otp, otc, otv, x) + 1e-300)
+    # adjoint identity y^T (A x) = (A^T y)^T x
+    z = np.random.default_rng(8).standard_normal(n)
+    lhs, rhs = z @ A.spmv(x), A.spmv(z, trans=True) @ x
+    assert abs(lhs - rhs) <= 1e-12 * (np.abs(z) @ P.row_abs_bound(ptr, col, val, x))
+    # FP32 values and vectors
+    x32 = x.astype(np.float32)
+    y32 = A.spmv(x32, precision=32)
+    yo32 = oracle.spmv_f32(ptr, col, val.astype(np.float32), x32)
+    b32 = 4 * m * EPS32 * P.row_abs_bound(ptr, col, val.astype(np.float32).astype(np.float64), x32) + 1e-30
+    assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
+
+
+def test_spmv_edge_cases(ctx, oracle):
+    # empty rows, a dense row, a single entry
+    n = 70
+    rows = [[], [0], list(range(n)), [5, 69], []] + [[i] for i in range(5, n)]
+    ptr = np.zeros(n + 1, np.int32)
+    cols = []
+    for i, r in enumerate(rows[:n]):
+        cols += r
+        ptr[i + 1] = len(cols)
+    col = np.array(cols, np.int32)
+    val = np.random.default_rng(3).standard_normal(len(col))
+    A = _mat(ctx, ptr, col, val, transpose=True)
+    x = np.random.default_rng(4).standard_normal(n)
+    y = A.spmv(x)
+    yo = oracle.spmv(ptr, col, val, x)
+    assert np.allclose(y, yo, rtol=1e-13, atol=1e-13)
+    assert y[0] == 0.0 and y[4] == 0.0
+    tp, tc, tv = A.transpose_csr()
+    otp, otc, otv = oracle.transpose(ptr, col, val)
+    assert np.array_equal(tp, otp) and np.array_equal(tc, otc) and np.array_equal(tv, otv)
+
+
+def test_bad_csr_rejected(ctx):
+    from solver_cpp_amd import api
+    ptr = np.array([0, 1, 2], np.int32)
+    with pytest.raises(api.KsolveError):
+        api.Matrix(ctx, ptr, np.array([0, 5], np.int32), np.ones(2))
+    with pytest.raises(api.KsolveError):
+        api.Matrix(ctx, np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32), np.ones(2))
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 100_003, 3_000_001])
+def test_dot_norm_axpy(ctx, n):
+    import math
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    d = ctx.dot(x, y)
+    exact = math.fsum(x * y)  # products exact to 1 ulp each
+    assert abs(d - exact) <= 64 * EPS64 * np.abs(x * y).sum() + 1e-300
+    nr = ctx.norm2(x)
+    assert abs(nr - math.sqrt(math.fsum(x * x))) <= 64 * EPS64 * nr
+    # closed form: sum_{i<n} i = n(n-1)/2 (exact in FP64 at these n)
+    i = np.arange(n, dtype=np.float64)
+    assert ctx.dot(i, np.ones(n)) == n * (n - 1) / 2
+    o = ctx.axpy(0.75, x, y)
+    assert np.allclose(o, 0.75 * x + y, rtol=0, atol=4 * EPS64 * (np.abs(0.75 * x) + np.abs(y)).max())
+    # FP32 inputs are accumulated in FP64
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    d32 = ctx.dot(x32, y32, precision=32)
+    assert abs(d32 - math.fsum(x32.astype(np.float64) * y32)) <= 64 * EPS64 * np.abs(x32 * y32.astype(np.float64)).sum() + 1e-300
+
+
+def test_deterministic_reductions(ctx):
+    x = np.random.default_rng(1).standard_normal(2_000_000)
+    vals = {ctx.dot(x, x) for _ in range(5)}
+    assert len(vals) == 1
+
+
+@pytest.mark.parametrize("k", [0, 3, 20, 40])
+def test_cgs_step(ctx, oracle, k):
+    n = 50_000
+    rng = np.random.default_rng(k)
+    V = np.linalg.qr(rng.standard_normal((n, k + 1)))[0].T.copy()
+    w = rng.standard_normal(n)
+    wg, hg = ctx.cgs(V, w)
+    wo, ho = oracle.cgs(V, w, k)
+    assert np.allclose(hg, ho, rtol=1e-12, atol=1e-12 * np.linalg.norm(w))
+    assert np.allclose(wg, wo, rtol=0, atol=1e-12 * np.linalg.norm(w))
+    # invariant: w is orthogonal to the basis afterwards
+    assert np.abs(V @ wg).max() <= 1e-12 * np.linalg.norm(w)
+
+
+@pytest.mark.parametrize("k", [1, 2, 4])
+def test_kskip_basis_and_gram(ctx, oracle, k):
+    ptr, col, val, _, _ = P.convdiff3d(10, pe=0.0)
+    n = len(ptr) - 1
+    rng = np.random.default_rng(k)
+    r, p = rng.standard_normal(n), rng.standard_normal(n)
+    A = _mat(ctx, ptr, col, val)
+    Ar, Ap = A.kskip_basis(k, r, p)
+    Aro, Apo = oracle.kskip_basis(ptr, col, val, k, r, p)
+    for G, O in ((Ar, Aro), (Ap, Apo)):
+        for i in range(G.shape[0]):
+            assert np.linalg.norm(G[i] - O[i]) <= 1e-13 * np.linalg.norm(O[i])
+    d, e, z = ctx.kskip_gram(k, r, p, Aro, Apo)
+    do, eo, zo = oracle.kskip_gram(k, r, p, Aro, Apo)
+    sc = np.linalg.norm(r) * np.linalg.norm(Apo, axis=1).max() + np.linalg.norm(p) * np.linalg.norm(Apo, axis=1).max()
+    for g, o in ((d, do), (e, eo), (z, zo)):
+        assert np.allclose(g, o, rtol=0, atol=1e-13 * sc)
+
+
+def test_sddmm(ctx, oracle):
+    ptr, col, val, _, _ = P.convdiff3d(9)
+    n = len(ptr) - 1
+    rng = np.random.default_rng(5)
+    u, v = rng.standard_normal(n), rng.standard_normal(n)
+    A = _mat(ctx, ptr, col, val)
+    g = A.sddmm(-1.0, u, v)
+    o = oracle.sddmm(ptr, col, -1.0, u, v)
+    assert np.array_equal(g, o)

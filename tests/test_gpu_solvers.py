@@ -1,0 +1,170 @@
+"""GPU solvers vs the CPU oracle on the same seeded inputs (through the C ABI).
+
+Parity bar: the reported iteration count (`loop =`) within +-1 of the
+oracle's, the same exit status, and x within a relative distance that the
+rounding-order difference allows (the GPU reduces in a different order and
+with FMA); residual histories agree to the same tolerance.
+"""
+import numpy as np
+import pytest
+
+from solver_cpp_amd import _abi
+from tests import _problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _mat(ctx, ptr, col, val, **kw):
+    from solver_cpp_amd import api
+    return api.Matrix(ctx, ptr, col, val, **kw)
+
+
+def _run(ctx, oracle, ptr, col, val, b, outer, inner="NO", xtol=1e-8, it_tol=1, **kw):
+    from solver_cpp_amd import api
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    x, r, h = api.solve(ctx, A, b, outer, inner, **kw)
+    xo, ro, ho = oracle.solve(ptr, col, val, b, outer, inner, **kw)
+    assert r["status"] == ro["status"], (r, ro)
+    assert abs(r["iters"] - ro["iters"]) <= it_tol, (r["iters"], ro["iters"])
+    if r["status"] == 0:
+        assert np.linalg.norm(x - xo) <= xtol * np.linalg.norm(xo), np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    m = min(len(h), len(ho))
+    assert m > 0
+    assert np.array_equal(h[:m, 0], ho[:m, 0])
+    return x, r, h, xo, ro, ho
+
+
+@pytest.mark.parametrize("g", [16, 48])
+def test_cg_poisson2d(ctx, oracle, g):
+    ptr, col, val, b, xt = P.poisson2d(g)
+    x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "cg", outer_eps=1e-12, outer_maxloop=5000)
+    assert r["status"] == 0 and r["relres_true"] < 1e-11
+    m = min(len(h), len(ho), 30)
+    assert np.allclose(h[:m, 1], ho[:m, 1], rtol=1e-8)
+
+
+def test_cg_maxloop_and_names(ctx, oracle):
+    ptr, col, val, b, xt = P.poisson2d(24)
+    # maxloop exit: loop = maxloop + 1, status 2 (cg.cpp:184-240)
+    x, r, h, *_ = _run(ctx, oracle, ptr, col, val, b, "CG", outer_eps=1e-14, outer_maxloop=7)
+    assert r["status"] == 2 and r["iters"] == 8 and len(h) == 7
+
+
+def test_bicg_convdiff(ctx, oracle):
+    ptr, col, val, b, xt = P.convdiff3d(12)
+    _run(ctx, oracle, ptr, col, val, b, "bicg", outer_eps=1e-10, outer_maxloop=2000)
+
+
+@pytest.mark.parametrize("compat", [0, _abi.KS_COMPAT_LAST_TERM])
+@pytest.mark.parametrize("R", [5, 31])
+def test_gmres_convdiff(ctx, oracle, compat, R):
+    ptr, col, val, b, xt = P.convdiff3d(12)
+    x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "gmres", outer_eps=1e-10, outer_maxloop=3000,
+                               outer_restart=R, ref_compat=compat)
+    if compat == 0:
+        assert r["relres_true"] < 2e-10
+
+
+def test_gmres_maxloop_exit(ctx, oracle):
+    ptr, col, val, b, xt = P.convdiff3d(10)
+    x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "gmres", outer_eps=1e-14, outer_maxloop=23,
+                               outer_restart=10)
+    assert r["status"] == 2 and r["iters"] == ro["iters"]
+
+
+@pytest.mark.parametrize("k", [1, 2, 4])
+def test_kskipcg_derived(ctx, oracle, k):
+    ptr, col, val, b, xt = P.convdiff3d(12, pe=0.0)
+    _run(ctx, oracle, ptr, col, val, b, "kskipcg", outer_eps=1e-10, outer_maxloop=3000, outer_kskip=k, it_tol=k + 1)
+
+
+def test_kskipcg_as_written(ctx, oracle):
+    # the reference's recurrences diverge; compare the first checks
+    ptr, col, val, b, xt = P.convdiff3d(10, pe=0.0)
+    from solver_cpp_amd import api
+    A = _mat(ctx, ptr, col, val)
+    kw = dict(outer_eps=1e-10, outer_maxloop=60, outer_kskip=2, ref_compat=_abi.KS_COMPAT_KSKIP_AS_WRITTEN)
+    x, r, h = api.solve(ctx, A, b, "kskipcg", **kw)
+    xo, ro, ho = oracle.solve(ptr, col, val, b, "kskipcg", **kw)
+    assert r["status"] == ro["status"] == 2 and r["iters"] == ro["iters"]
+    assert np.allclose(h[:4, 1], ho[:4, 1], rtol=1e-6)
+
+
+def test_kskipbicg(ctx, oracle):
+    ptr, col, val, b, xt = P.convdiff3d(12)
+    _run(ctx, oracle, ptr, col, val, b, "kskipbicg", outer_eps=1e-10, outer_maxloop=3000, outer_kskip=3, it_tol=4)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_vpgmres_gmres_inner(ctx, oracle, prec):
+    ptr, col, val, b, xt = P.convdiff3d(14)
+    x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "vpgmres", "gmres", outer_eps=1e-10,
+                               outer_maxloop=500, outer_restart=21, inner_maxloop=11, inner_restart=12,
+                               inner_precision=prec, xtol=1e-7)
+    assert r["status"] == 0 and r["relres_true"] < 2e-10
+    assert r["inner_steps"] > 0
+
+
+def test_vpgmres_ref_last_term(ctx, oracle):
+    ptr, col, val, b, xt = P.convdiff3d(8)
+    from solver_cpp_amd import api
+    A = _mat(ctx, ptr, col, val)
+    kw = dict(outer_eps=1e-10, outer_maxloop=40, outer_restart=21, inner_maxloop=11, inner_restart=12,
+              ref_compat=_abi.KS_COMPAT_LAST_TERM)
+    x, r, h = api.solve(ctx, A, b, "vpgmres", "gmres", **kw)
+    xo, ro, ho = oracle.solve(ptr, col, val, b, "vpgmres", "gmres", **kw)
+    assert r["status"] == ro["status"] and abs(r["iters"] - ro["iters"]) <= 1
+    assert np.allclose(h[:10, 1], ho[:10, 1], rtol=1e-6)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_vpcg_cg_inner(ctx, oracle, prec):
+    ptr, col, val, b, xt = P.convdiff3d(14, pe=0.0)
+    x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "vpcg", "cg", outer_eps=1e-10, outer_maxloop=500,
+                               inner_maxloop=10, inner_precision=prec, xtol=1e-7)
+    assert r["status"] == 0
+
+
+def test_vpcg_fletcher_reeves_compat(ctx, oracle):
+    ptr, col, val, b, xt = P.convdiff3d(8, pe=0.0)
+    from solver_cpp_amd import api
+    A = _mat(ctx, ptr, col, val)
+    kw = dict(outer_eps=1e-10, outer_maxloop=30, inner_maxloop=10, ref_compat=_abi.KS_COMPAT_VPCG_FR)
+    x, r, h = api.solve(ctx, A, b, "vpcg", "cg", **kw)
+    xo, ro, ho = oracle.solve(ptr, col, val, b, "vpcg", "cg", **kw)
+    assert r["status"] == ro["status"] and r["iters"] == ro["iters"]
+    assert np.allclose(h[:10, 1], ho[:10, 1], rtol=1e-6)
+
+
+def test_zero_rhs(ctx):
+    from solver_cpp_amd import api
+    ptr, col, val, b, xt = P.poisson2d(8)
+    A = _mat(ctx, ptr, col, val)
+    for name in ("cg", "gmres", "bicg"):
+        x, r, h = api.solve(ctx, A, np.zeros(len(b)), name, outer_restart=10)
+        assert r["status"] == 0 and np.all(x == 0)
+
+
+def test_x0_given(ctx, oracle):
+    from solver_cpp_amd import api
+    ptr, col, val, b, xt = P.poisson2d(20)
+    A = _mat(ctx, ptr, col, val)
+    x0 = xt + 1e-3 * np.random.default_rng(0).standard_normal(len(xt))
+    x, r, h = api.solve(ctx, A, b, "cg", x0=x0, outer_eps=1e-12)
+    xo, ro, ho = oracle.solve(ptr, col, val, b, "cg", x0=x0, outer_eps=1e-12)
+    assert r["status"] == 0 and abs(r["iters"] - ro["iters"]) <= 1
+
+
+def test_adjoint_small(ctx, oracle):
+    from solver_cpp_amd import api
+    ptr, col, val, b, xt = P.convdiff3d(9)
+    n = len(ptr) - 1
+    A = _mat(ctx, ptr, col, val, transpose=True)
+    x, r, _ = api.solve(ctx, A, b, "gmres", outer_eps=1e-13, outer_restart=40, outer_maxloop=2000)
+    g = np.random.default_rng(2).standard_normal(n)
+    gb, gv, ra = api.adjoint(ctx, A, x, g, "gmres", outer_eps=1e-13, outer_restart=40, outer_maxloop=2000)
+    D = P.dense(ptr, col, val)
+    lam = np.linalg.solve(D.T, g)
+    assert np.linalg.norm(gb - lam) <= 1e-9 * np.linalg.norm(lam)
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    assert np.allclose(gv, -lam[rows] * x[col], rtol=0, atol=1e-9 * np.abs(lam).max() * np.abs(x).max())
