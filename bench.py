@@ -1,0 +1,142 @@
+#!/usr/bin/env python3
+"""Benchmark: mixed-precision VPGMRES solve time on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 3-D 7-point
+Poisson on a 160^3 grid (n = 4,096,000, nnz = 28,518,400), b ~ U(-1,1)
+(seed 3), x0 = 0; outer FP64 flexible GMRES(m=20) (-R 21), inner 10 steps of
+FP32 GMRES on an FP32 copy of A (-s gmres -l 11 -r 12 -e 0), stop at 1e-10
+relative residual.  One step = one full solve from x0 = 0 with inputs
+resident in HBM.
+
+Reported with it (outside the timed region): plain FP64 GMRES(20) on the
+same GPU (speed-up), the relative error of the mixed solution against an FP64
+GMRES solution run to 1e-12, SpMV bandwidth, the roofline of the dominant
+kernel (hipEvents around its launches inside the timed solves), and the CPU
+oracle timed on a bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "solve s to 1e-10 rel-res, mixed vs FP64 on GPU; SpMV HBM GB/s; adjoint grad s"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=160, help="Poisson grid edge (160 = the benchmark)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GMRES comparison")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="outer steps in the CPU sample")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from solver_cpp_amd import api, gen
+    from solver_cpp_amd._abi import KERNEL_CLASSES
+
+    g = a.grid
+    P = gen.problem("poisson3d", scale=g)
+    n, nnz = len(P["ptr"]) - 1, len(P["col"])
+    ctx = api.Context(0)
+    A = api.Matrix(ctx, P["ptr"], P["col"], P["val"], fp32=True)
+    b = torch.from_numpy(P["b"]).cuda()
+    mixed = dict(outer_restart=21, outer_maxloop=5000, outer_eps=1e-10, inner_maxloop=11, inner_restart=12,
+                 inner_eps=0.0, inner_precision=32)
+    dom = KERNEL_CLASSES.index("spmv32")
+
+    def run_mixed(profile=0):
+        return api.solve(ctx, A, b, "vpgmres", "gmres", history=False, profile=profile, **mixed)
+
+    for _ in range(a.warmup):
+        run_mixed()
+    torch.cuda.synchronize()
+    res = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        x, r, _ = run_mixed(profile=1 << dom)
+        res.append(r)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sec = wall / a.steps
+    r0 = res[-1]
+    # dominant kernel: mean duration of its launches inside the timed solves
+    ms_tot = sum(rr["prof"].get(KERNEL_CLASSES[dom], {}).get("ms", 0.0) for rr in res)
+    launches = sum(rr["prof"].get(KERNEL_CLASSES[dom], {}).get("launches", 0) for rr in res)
+    bytes_per = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n
+    ms_per = ms_tot / max(launches, 1)
+    achieved = bytes_per / (ms_per * 1e-3) / 1e9 if launches else None
+
+    out = {
+        "metric": METRIC, "value": round(sec, 6), "unit": "s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 outer / f32 inner", "data": "synthetic",
+        "config": {"workload": f"VPGMRES(20)+10xFP32 GMRES, 3-D 7-pt Poisson {g}^3, tol 1e-10",
+                   "n": n, "nnz": nnz, "outer_steps": r0["steps"], "inner_steps": r0["inner_steps"],
+                   "iters": r0["iters"], "relres_true": r0["relres_true"]},
+    }
+    out["roofline"] = {"bound": "hbm", "kernel": "k_spmv<float> (inner FP32 SpMV)",
+                       "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                       "traffic": None, "bytes_per_launch": bytes_per, "ms_per_launch": ms_per,
+                       "launches": launches}
+    out["solve_bytes_GBps"] = round(r0["bytes"] / r0["seconds"] / 1e9, 1)
+
+    if not a.no_fp64:
+        # FP64 GMRES(20) on the same GPU, and the FP64 reference solution
+        x64, r64, _ = api.solve(ctx, A, b, "gmres", history=False, outer_restart=21, outer_maxloop=200000,
+                                outer_eps=1e-10)
+        xo, ro, _ = api.solve(ctx, A, b, "gmres", history=False, outer_restart=21, outer_maxloop=400000,
+                              outer_eps=1e-12)
+        err = float(torch.linalg.norm(x - xo) / torch.linalg.norm(xo))
+        out["fp64_gmres"] = {"seconds": round(r64["seconds"], 4), "iters": r64["iters"],
+                             "relres_true": r64["relres_true"]}
+        out["speedup_vs_fp64_gmres"] = round(r64["seconds"] / sec, 3)
+        out["rel_error_vs_fp64_oracle"] = err
+    # SpMV bandwidth (algorithmic bytes / mean launch time)
+    spmv = {}
+    for prec in (64, 32):
+        dt = torch.float64 if prec == 64 else torch.float32
+        xx = torch.randn(n, dtype=dt, device="cuda")
+        yy = torch.empty_like(xx)
+        A.spmv_bench(xx, yy, reps=5, precision=prec)
+        ms = A.spmv_bench(xx, yy, reps=50, precision=prec)
+        sv = prec // 8
+        by = (sv + 4) * nnz + 4 * (n + 1) + 2 * sv * n
+        spmv[f"fp{prec}"] = {"us": round(ms * 1e3, 2), "GBps": round(by / (ms * 1e-3) / 1e9, 1)}
+    out["spmv"] = spmv
+
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        kw = dict(mixed)
+        kw["outer_maxloop"] = a.cpu_steps + 1  # count+1 >= maxloop stops after cpu_steps steps
+        kw["omp_threads"] = threads
+        bnp = P["b"]
+        t = time.perf_counter()
+        xc, rc, _ = O.solve(P["ptr"], P["col"], P["val"], bnp, "vpgmres", "gmres", history=False, **kw)
+        tc = time.perf_counter() - t
+        per_step = tc / max(rc["steps"], 1)
+        out["cpu_baseline"] = {"value": round(per_step * r0["steps"], 3), "unit": "s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle VPGMRES (FP32 inner) {rc['steps']} outer steps timed "
+                                         f"({tc:.2f} s), scaled to the GPU's {r0['steps']} outer steps"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -150,6 +150,7 @@ ks_status ks_mat_update_values(ks_mat* m, const double* val, uint32_t flags);
 /* Copy the device-built A^T (CSR of the transpose) out, for checking. */
 ks_status ks_mat_get_transpose(ks_mat* m, int32_t* tptr, int32_t* tcol,
                                double* tval, uint32_t flags);
+/* Destroy every matrix of a context before the context itself. */
 void ks_mat_destroy(ks_mat* m);
 /* n, nnz and the SpMV variant chosen (0 vector, 1 merge-path) with its lanes/row */
 ks_status ks_mat_info(const ks_mat* m, int64_t* n, int64_t* nnz,

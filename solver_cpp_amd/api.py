@@ -17,6 +17,7 @@ Arrays may be numpy arrays (host memory; copied to and from HBM) or torch
 tensors on the GPU (used in place).
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -87,6 +88,7 @@ class Context:
                                   f"{self.lib.ks_status_string(st).decode()}")
         self.h = h
         self.device = device
+        self._mats = weakref.WeakSet()  # matrices must be destroyed before the context
 
     def check(self, st, what):
         if st != 0:
@@ -102,6 +104,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for m in list(self._mats):
+                m.close()
             self.lib.ks_ctx_destroy(self.h)
             self.h = None
 
@@ -187,6 +191,7 @@ class Matrix:
                   "ks_mat_create_csr")
         self.h = h
         self.has_t = transpose
+        ctx._mats.add(self)
 
     def info(self):
         n, nnz, kind, lanes = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
@@ -247,7 +252,8 @@ class Matrix:
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx.lib.ks_mat_destroy(self.h)
+            if getattr(self.ctx, "h", None):  # context still alive
+                self.ctx.lib.ks_mat_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -125,7 +125,6 @@ struct ks_ctx {
   bool own_stream = true;
   std::string err;
   ks::DBuf<double> part;                 // kMaxGrid * kMaxNV reduction partials
-  ks::DBuf<unsigned> counter;            // reduction ticket
   ks::DBuf<unsigned long long> stats;    // launches[8], bytes[8]
   ks::DBuf<double> scal;                 // scalar outputs of the BLAS entry points
   int* hflags = nullptr;                 // host-mapped flags (host view)
@@ -137,8 +136,13 @@ struct ks_ctx {
 
 namespace ks {
 
+// SpMV plan of one CSR matrix: CSR-stream tiles (kind 0) or L lanes per
+// row (kind 1).
 struct Plan {
-  int L = 8;  // lanes per row
+  int kind = 0;
+  int L = 8;
+  int64_t ntiles = 0;
+  DBuf<int32_t> tiles;  // tile row starts, ntiles + 1
 };
 
 template <class T>
@@ -147,7 +151,10 @@ struct MatView {
   const int32_t* ptr = nullptr;
   const int32_t* col = nullptr;
   const T* val = nullptr;
+  int kind = 0;
   int L = 8;
+  int64_t ntiles = 0;
+  const int32_t* tiles = nullptr;
 };
 
 }  // namespace ks
@@ -174,22 +181,26 @@ template <>
 inline ks::MatView<double> ks_mat::view<double>(bool trans) const {
   ks::MatView<double> v;
   v.n = n; v.nnz = nnz;
-  if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval.p; v.L = tplan.L; }
-  else { v.ptr = ptr.p; v.col = col.p; v.val = val.p; v.L = plan.L; }
+  const ks::Plan& pl = trans ? tplan : plan;
+  if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval.p; }
+  else { v.ptr = ptr.p; v.col = col.p; v.val = val.p; }
+  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p;
   return v;
 }
 template <>
 inline ks::MatView<float> ks_mat::view<float>(bool trans) const {
   ks::MatView<float> v;
   v.n = n; v.nnz = nnz;
-  if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval32.p; v.L = tplan.L; }
-  else { v.ptr = ptr.p; v.col = col.p; v.val = val32.p; v.L = plan.L; }
+  const ks::Plan& pl = trans ? tplan : plan;
+  if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval32.p; }
+  else { v.ptr = ptr.p; v.col = col.p; v.val = val32.p; }
+  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p;
   return v;
 }
 
 namespace ks {
 
-inline RedArgs red(ks_ctx* c) { return RedArgs{c->part.p, c->counter.p}; }
+inline RedArgs red(ks_ctx* c) { return RedArgs{c->part.p}; }
 inline StatArgs stat(ks_ctx* c, int cls, double bytes) {
   return StatArgs{c->stats.p, c->stats.p + KS_K_NCLASS, cls,
                   (unsigned long long)(bytes > 0 ? bytes : 0)};
@@ -209,6 +220,13 @@ inline int grid_rows(int64_t nrb) {
   return (int)g;
 }
 
+// Finisher of a reduction whose producer ran on G workgroups.
+template <int NV, class F>
+void launch_finish(ks_ctx* c, int G, const F& f, Gate g) {
+  k_finish<NV, F><<<1, kBlock, 0, c->stream>>>(G, c->part.p, f, g);
+  KS_HIPCK(hipGetLastError());
+}
+
 template <int NV, class Op>
 void launch_map(ks_ctx* c, int64_t n, const Op& op, Gate g, int cls, double bytes) {
   const int G = grid_map(n);
@@ -216,6 +234,7 @@ void launch_map(ks_ctx* c, int64_t n, const Op& op, Gate g, int cls, double byte
   k_map<NV, Op><<<G, kBlock, 0, c->stream>>>(n, op, g, red(c), stat(c, cls, bytes));
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
+  if constexpr (NV > 0) launch_finish<NV>(c, G, op, g);
 }
 
 template <class Op>
@@ -231,7 +250,7 @@ inline double spmv_bytes(int64_t n, int64_t nnz) {
 }
 
 template <class T, int L, int NV, class Epi>
-void spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, int cls, double bytes) {
+int spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, int cls, double bytes) {
   constexpr int RPB = kBlock / L;
   const int64_t nrb = (A.n + RPB - 1) / RPB;
   const int G = grid_rows(nrb);
@@ -240,45 +259,80 @@ void spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, 
                                                      stat(c, cls, bytes));
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
+  return G;
 }
 
-// y-side epilogue of SpMV plus NV reduction values.
+template <class T, int NX, int NV, class Epi>
+int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
+                double bytes) {
+  const int G = grid_rows(A.ntiles);
+  c->prof.begin(c->stream, cls);
+  k_spmv_stream<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.ptr, A.col, A.val, x0, x1,
+                                                             epi, g, red(c), stat(c, cls, bytes));
+  c->prof.end(c->stream);
+  KS_HIPCK(hipGetLastError());
+  return G;
+}
+
+// y-side epilogue of SpMV plus NV reduction values (finished by k_finish).
 template <class T, int NV, class Epi>
 void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, double extra_bytes = 0) {
   const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
   const double bytes = spmv_bytes<T>(A.n, A.nnz) + extra_bytes;
-  switch (A.L) {
-    case 2: spmv_L<T, 2, NV>(c, A, x, epi, g, cls, bytes); break;
-    case 4: spmv_L<T, 4, NV>(c, A, x, epi, g, cls, bytes); break;
-    case 8: spmv_L<T, 8, NV>(c, A, x, epi, g, cls, bytes); break;
-    case 16: spmv_L<T, 16, NV>(c, A, x, epi, g, cls, bytes); break;
-    case 32: spmv_L<T, 32, NV>(c, A, x, epi, g, cls, bytes); break;
-    default: spmv_L<T, 64, NV>(c, A, x, epi, g, cls, bytes); break;
+  int G;
+  if (A.kind == 0) {
+    G = spmv_stream<T, 1, NV>(c, A, x, x, epi, g, cls, bytes);
+  } else {
+    switch (A.L) {
+      case 2: G = spmv_L<T, 2, NV>(c, A, x, epi, g, cls, bytes); break;
+      case 4: G = spmv_L<T, 4, NV>(c, A, x, epi, g, cls, bytes); break;
+      case 8: G = spmv_L<T, 8, NV>(c, A, x, epi, g, cls, bytes); break;
+      case 16: G = spmv_L<T, 16, NV>(c, A, x, epi, g, cls, bytes); break;
+      case 32: G = spmv_L<T, 32, NV>(c, A, x, epi, g, cls, bytes); break;
+      default: G = spmv_L<T, 64, NV>(c, A, x, epi, g, cls, bytes); break;
+    }
   }
+  if constexpr (NV > 0) launch_finish<NV>(c, G, epi, g);
 }
 
-template <class T, int L>
-void spmv2_L(ks_ctx* c, const MatView<T>& A, const T* x1, const T* x2, T* y1, T* y2, Gate g) {
-  constexpr int RPB = kBlock / L;
-  const int64_t nrb = (A.n + RPB - 1) / RPB;
-  const int G = grid_rows(nrb);
-  const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * A.n;
-  c->prof.begin(c->stream, KS_K_BASIS);
-  k_spmv2<T, L><<<G, kBlock, 0, c->stream>>>(A.n, A.ptr, A.col, A.val, x1, x2, y1, y2, g,
-                                            stat(c, KS_K_BASIS, bytes));
-  c->prof.end(c->stream);
-  KS_HIPCK(hipGetLastError());
-}
+template <class T>
+struct EpiStore2;
+
+// y1 = A x1 and y2 = A x2 in one pass over the matrix (always CSR-stream).
 template <class T>
 void launch_spmv2(ks_ctx* c, const MatView<T>& A, const T* x1, const T* x2, T* y1, T* y2, Gate g) {
-  switch (A.L) {
-    case 2: spmv2_L<T, 2>(c, A, x1, x2, y1, y2, g); break;
-    case 4: spmv2_L<T, 4>(c, A, x1, x2, y1, y2, g); break;
-    case 8: spmv2_L<T, 8>(c, A, x1, x2, y1, y2, g); break;
-    case 16: spmv2_L<T, 16>(c, A, x1, x2, y1, y2, g); break;
-    case 32: spmv2_L<T, 32>(c, A, x1, x2, y1, y2, g); break;
-    default: spmv2_L<T, 64>(c, A, x1, x2, y1, y2, g); break;
+  const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * A.n;
+  spmv_stream<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);
+}
+
+// CSR-stream tiles from host row pointers.
+inline std::vector<int32_t> build_tiles(const int32_t* hp, int64_t n) {
+  std::vector<int32_t> t;
+  t.reserve(n / 64 + 2);
+  t.push_back(0);
+  int64_t r = 0;
+  while (r < n) {
+    const int64_t start = r, j0 = hp[r];
+    if (hp[r + 1] - j0 > kTileNnz) {
+      ++r;
+    } else {
+      while (r < n && r - start < kTileRows && hp[r + 1] - j0 <= kTileNnz) ++r;
+    }
+    t.push_back((int32_t)r);
   }
+  return t;
+}
+
+inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t nnz, bool vector) {
+  pl.L = 2;
+  const double mean = n > 0 ? (double)nnz / (double)n : 1.0;
+  while (pl.L < 64 && pl.L < mean) pl.L *= 2;
+  pl.kind = vector ? 1 : 0;
+  std::vector<int32_t> t = build_tiles(hp, n);
+  pl.ntiles = (int64_t)t.size() - 1;
+  pl.tiles.alloc(t.size());
+  KS_HIPCK(hipMemcpyAsync(pl.tiles.p, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice, c->stream));
+  KS_HIPCK(hipStreamSynchronize(c->stream));
 }
 
 // Lanes per row: smallest power of two >= mean row length, in [2, 64].
@@ -294,8 +348,15 @@ template <class T>
 struct EpiStore {
   T* y;
   __device__ void prep() {}
-  __device__ void row(int64_t i, T s, double*) { y[i] = s; }
-  __device__ void fin(const double*) {}
+  __device__ void row(int64_t i, T s, T, double*) { y[i] = s; }
+  __device__ void fin(const double*) const {}
+};
+template <class T>
+struct EpiStore2 {
+  T *y0, *y1;
+  __device__ void prep() {}
+  __device__ void row(int64_t i, T s0, T s1, double*) { y0[i] = s0; y1[i] = s1; }
+  __device__ void fin(const double*) const {}
 };
 
 // y = b - A x (b FP64), acc += y^2, Fin on the total.
@@ -305,12 +366,12 @@ struct EpiResid {
   T* y;
   Fin f;
   __device__ void prep() {}
-  __device__ void row(int64_t i, T s, double* acc) {
+  __device__ void row(int64_t i, T s, T, double* acc) {
     const T r = (T)(b[i] - (double)s);
     y[i] = r;
     acc[0] += (double)r * (double)r;
   }
-  __device__ void fin(const double* s) { f(s); }
+  __device__ void fin(const double* s) const { f(s); }
 };
 
 // y = A x, acc += d . y, Fin on the total.
@@ -320,11 +381,11 @@ struct EpiDot {
   T* y;
   Fin f;
   __device__ void prep() {}
-  __device__ void row(int64_t i, T s, double* acc) {
+  __device__ void row(int64_t i, T s, T, double* acc) {
     y[i] = s;
     acc[0] += (double)d[i] * (double)s;
   }
-  __device__ void fin(const double* s) { f(s); }
+  __device__ void fin(const double* s) const { f(s); }
 };
 
 struct FinNone {

@@ -155,7 +155,7 @@ void build_transpose(ks_mat* m) {
     k_gather<double, double><<<grid_map(nnz), kBlock, 0, s>>>(nnz, m->tperm.p, m->val.p, m->tval.p);
     KS_HIPCK(hipGetLastError());
   }
-  m->tplan.L = choose_lanes(n, nnz);
+  make_plan(c, m->tplan, htp.data(), n, nnz, (m->flags & KS_MAT_VECTOR) != 0);
   m->has_t = true;
 }
 
@@ -408,8 +408,6 @@ ks_status ks_ctx_create(int32_t device, ks_ctx** out) {
     KS_HIPCK(hipSetDevice(device));
     KS_HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->part.alloc((size_t)kMaxGrid * kMaxNV);
-    c->counter.alloc(1);
-    KS_HIPCK(hipMemsetAsync(c->counter.p, 0, sizeof(unsigned), c->stream));
     c->stats.alloc(2 * KS_K_NCLASS);
     c->scal.alloc(1024);
     KS_HIPCK(hipHostMalloc((void**)&c->hflags, 64 * sizeof(int), hipHostMallocMapped));
@@ -435,7 +433,6 @@ void ks_ctx_destroy(ks_ctx* c) {
   c->prof.destroy();
   for (auto e : c->evpool) (void)hipEventDestroy(e);
   c->part.release();
-  c->counter.release();
   c->stats.release();
   c->scal.release();
   if (c->hflags) (void)hipHostFree(c->hflags);
@@ -491,8 +488,7 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   for (int64_t j = 0; j < nnz; ++j)
     KS_CHECK(hc[j] >= 0 && hc[j] < n, KS_ERR_BAD_INPUT, "column index out of range at entry " + std::to_string(j));
   m->max_row = mx;
-  m->plan.L = choose_lanes(n, nnz);
-  if (flags & KS_MAT_VECTOR) m->plan.L = choose_lanes(n, nnz);
+  make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0);
   if (flags & KS_MAT_TRANSPOSE) build_transpose(m.get());
   if (flags & KS_MAT_FP32_COPY) ensure_fp32(m.get());
   KS_HIPCK(hipStreamSynchronize(c->stream));
@@ -554,7 +550,7 @@ ks_status ks_mat_info(const ks_mat* m, int64_t* n, int64_t* nnz, int32_t* kind, 
   if (!m) return KS_ERR_BAD_INPUT;
   if (n) *n = m->n;
   if (nnz) *nnz = m->nnz;
-  if (kind) *kind = 0;
+  if (kind) *kind = m->plan.kind;
   if (lanes) *lanes = m->plan.L;
   return KS_OK;
 }

@@ -74,10 +74,10 @@ struct Gate {
   __device__ bool off() const { return (a && *a) || (b && *b); }
 };
 
-// Reduction scratch of the context: per-workgroup partials and a ticket.
+// Reduction scratch of the context: partial sums, value-major with stride
+// kMaxGrid: part[i * kMaxGrid + workgroup].
 struct RedArgs {
   double* part;
-  unsigned* counter;
 };
 
 // Executed-launch statistics per kernel class (counted by workgroup 0).
@@ -101,14 +101,13 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Sums acc[0..NV) over the grid.  Returns true in the last workgroup to
-// finish, where out[0..NV) (LDS) then holds the totals for every thread.
-// Deterministic: partials are written per workgroup and summed by the last
-// workgroup in a fixed order, independent of completion order.
+// Stores this workgroup's sums of acc[0..NV) as its partials.  The totals
+// are formed by k_finish, launched right after the producing kernel: the
+// kernel boundary orders the partials before their use, so no device-scope
+// fence (which on gfx950 writes back and invalidates the XCD's L2) is needed.
 template <int NV>
-__device__ bool grid_sum(double (&acc)[NV], RedArgs ra, double* out) {
+__device__ void block_store(double (&acc)[NV], RedArgs ra) {
   __shared__ double sw[NV][kBlock / 64];
-  __shared__ int s_last;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int nw = kBlock / 64;
 #pragma unroll
@@ -117,24 +116,26 @@ __device__ bool grid_sum(double (&acc)[NV], RedArgs ra, double* out) {
     if (lane == 0) sw[i][w] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < NV; ++i) {
-      double s = 0.0;
+  if (threadIdx.x < NV) {
+    double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < nw; ++k) s += sw[i][k];
-      __hip_atomic_store(ra.part + (size_t)blockIdx.x * NV + i, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __threadfence();
-    unsigned t = atomicAdd(ra.counter, 1u);
-    s_last = (t == gridDim.x - 1);
+    for (int k = 0; k < nw; ++k) s += sw[threadIdx.x][k];
+    ra.part[(size_t)threadIdx.x * kMaxGrid + blockIdx.x] = s;
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
+}
+
+// Totals of the G partials of each of the NV values, summed in a fixed order
+// (deterministic), then f.fin(sums) on one thread.
+template <int NV, class F>
+__global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restrict__ part, F f, Gate g) {
+  if (g.off()) return;
+  __shared__ double sw[NV][kBlock / 64];
+  __shared__ double sums[NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int nw = kBlock / 64;
   for (int i = 0; i < NV; ++i) {
     double s = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-      s += __hip_atomic_load(ra.part + (size_t)b * NV + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < G; b += kBlock) s += part[(size_t)i * kMaxGrid + b];
     s = wave_sum(s);
     if (lane == 0) sw[i][w] = s;
   }
@@ -143,11 +144,10 @@ __device__ bool grid_sum(double (&acc)[NV], RedArgs ra, double* out) {
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < nw; ++k) s += sw[threadIdx.x][k];
-    out[threadIdx.x] = s;
+    sums[threadIdx.x] = s;
   }
-  if (threadIdx.x == 0) __hip_atomic_store(ra.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  return true;
+  if (threadIdx.x == 0) f.fin(sums);
 }
 
 // ------------------------------------------------------- generic map ----
@@ -170,12 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_map(int64_t n, Op op, Gate g, RedArg
     op.elem(i + 3 * stride, acc);
   }
   for (; i < n; i += stride) op.elem(i, acc);
-  if constexpr (NV > 0) {
-    __shared__ double sums[NV];
-    if (grid_sum<NV>(acc, ra, sums)) {
-      if (threadIdx.x == 0) op.fin(sums);
-    }
-  }
+  if constexpr (NV > 0) block_store<NV>(acc, ra);
 }
 
 // One-thread kernel for scalar recurrences.
@@ -242,14 +237,122 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
     for (int j = s + lane; j < e; j += L) sum += val[j] * x[col[j]];
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, L);
-    if (lane == 0 && row < n) epi.row(row, sum, acc);
+    if (lane == 0 && row < n) epi.row(row, sum, (T)0, acc);
   }
-  if constexpr (NV > 0) {
-    __shared__ double sums[NV];
-    if (grid_sum<NV>(acc, ra, sums)) {
-      if (threadIdx.x == 0) epi.fin(sums);
+  if constexpr (NV > 0) block_store<NV>(acc, ra);
+}
+
+// CSR-stream SpMV.  The rows are cut (on the host, once per matrix) into
+// tiles of at most kTileRows rows and kTileNnz entries; a row longer than
+// kTileNnz is a tile of its own.  Per tile the workgroup (1) streams the
+// tile's values and column indices with independent loads (eight per thread
+// in flight) and gathers x, writing the products to LDS, then (2) sums each
+// row's products from LDS with `tpr` threads per row (tpr = 1 when the tile
+// is full of short rows: a sequential sum in CSR order, the reference's
+// blas::MtxVec_mult order, blas.cpp:56-62).  NX = 2 forms A x0 and A x1 from
+// one pass over the matrix (the k-skip matrix-powers basis).
+constexpr int kTileNnz = 2048;
+constexpr int kTileRows = 256;
+constexpr int kTileUnroll = kTileNnz / kBlock;
+
+template <class T>
+__device__ inline T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+
+template <class T, int NX, int NV, class Epi>
+__global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
+                                                        const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const T* __restrict__ val, const T* __restrict__ x0,
+                                                        const T* __restrict__ x1, Epi epi, Gate g, RedArgs ra,
+                                                        StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  epi.prep();
+  __shared__ T prod[NX][kTileNnz];
+  __shared__ int32_t sptr[kTileRows + 1];
+  __shared__ double slong[NX][kBlock / 64];
+  double acc[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < (NV > 0 ? NV : 1); ++i) acc[i] = 0.0;
+  const int tid = threadIdx.x;
+  RowBlocks rbs{ntiles};
+  int64_t t, end, step;
+  rbs.range(t, end, step);
+  for (; t < end; t += step) {
+    const int32_t r0 = tiles[t], r1 = tiles[t + 1];
+    const int nr = r1 - r0;
+    const int32_t j0 = ptr[r0];
+    const int cnt = ptr[r1] - j0;
+    __syncthreads();  // LDS of the previous tile is free
+    if (cnt > kTileNnz) {
+      // one long row: strided partial sums, then a fixed tree
+      T s0 = 0, s1 = 0;
+      for (int k = tid; k < cnt; k += kBlock) {
+        const T a = ld_stream(val + j0 + k);
+        const int c = ld_stream(col + j0 + k);
+        s0 += a * x0[c];
+        if (NX == 2) s1 += a * x1[c];
+      }
+      const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        if (NX == 2) s1 += __shfl_xor(s1, off, 64);
+      }
+      if (lane == 0) {
+        slong[0][w] = (double)s0;
+        if (NX == 2) slong[NX - 1][w] = (double)s1;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        T a0 = 0, a1 = 0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+          a0 += (T)slong[0][k];
+          if (NX == 2) a1 += (T)slong[NX - 1][k];
+        }
+        epi.row(r0, a0, a1, acc);
+      }
+      continue;
+    }
+    for (int q = tid; q <= nr; q += kBlock) sptr[q] = ptr[r0 + q] - j0;
+    T v[kTileUnroll];
+    int c[kTileUnroll];
+#pragma unroll
+    for (int q = 0; q < kTileUnroll; ++q) {
+      const int k = tid + q * kBlock;
+      if (k < cnt) {
+        v[q] = ld_stream(val + j0 + k);
+        c[q] = ld_stream(col + j0 + k);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kTileUnroll; ++q) {
+      const int k = tid + q * kBlock;
+      if (k < cnt) {
+        prod[0][k] = v[q] * x0[c[q]];
+        if (NX == 2) prod[NX - 1][k] = v[q] * x1[c[q]];
+      }
+    }
+    __syncthreads();
+    // more than one thread per row only for rows averaging > 8 entries
+    int tpr = 1;
+    while (tpr < 64 && tpr * 2 * nr <= kBlock && tpr * 8 * nr < cnt) tpr *= 2;
+    const int grp = tid / tpr, lane = tid % tpr;
+    if (grp < nr) {
+      const int s = sptr[grp], e = sptr[grp + 1];
+      T a0 = 0, a1 = 0;
+      for (int k = s + lane; k < e; k += tpr) {
+        a0 += prod[0][k];
+        if (NX == 2) a1 += prod[NX - 1][k];
+      }
+      for (int off = tpr >> 1; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        if (NX == 2) a1 += __shfl_xor(a1, off, 64);
+      }
+      if (lane == 0) epi.row(r0 + grp, a0, a1, acc);
     }
   }
+  if constexpr (NV > 0) block_store<NV>(acc, ra);
 }
 
 // Two products with one pass over the matrix: y1 = A x1, y2 = A x2
@@ -332,31 +435,42 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(int64_t n, const int32_t* __re
 // h[i] = (w, V_{i0+i}) for i < nv (nv <= G): one pass over w and nv columns.
 template <class T, int G>
 __global__ __launch_bounds__(kBlock) void k_proj(int64_t n, const T* __restrict__ V, int64_t ldv, int nv,
-                                                 const T* __restrict__ w, double* hout, int ldh, Gate g,
-                                                 RedArgs ra, StatArgs st) {
+                                                 const T* __restrict__ w, Gate g, RedArgs ra, StatArgs st) {
   if (g.off()) return;
   st.count();
   double acc[G];
 #pragma unroll
   for (int v = 0; v < G; ++v) acc[v] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + stride < n; i += 2 * stride) {
+    const double wa = (double)w[i], wb = (double)w[i + stride];
+#pragma unroll
+    for (int v = 0; v < G; ++v)
+      if (v < nv) acc[v] += wa * (double)V[(int64_t)v * ldv + i] + wb * (double)V[(int64_t)v * ldv + i + stride];
+  }
+  for (; i < n; i += stride) {
     const double wi = (double)w[i];
 #pragma unroll
     for (int v = 0; v < G; ++v)
       if (v < nv) acc[v] += wi * (double)V[(int64_t)v * ldv + i];
   }
-  __shared__ double sums[G];
-  if (grid_sum<G>(acc, ra, sums)) {
-    if (threadIdx.x < nv) hout[(int64_t)threadIdx.x * ldh] = sums[threadIdx.x];
-  }
+  block_store<G>(acc, ra);
 }
+// Finisher of k_proj: h column entries.
+struct FinProj {
+  double* hout;
+  int ldh, nv;
+  __device__ void fin(const double* s) const {
+    for (int v = 0; v < nv; ++v) hout[(int64_t)v * ldh] = s[v];
+  }
+};
 
 // w -= sum_{i<nv} h_i V_i (classical GS update), then ||w||^2 -> fin.
-template <class T, class Fin>
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __restrict__ V, int64_t ldv,
                                                        int nv, T* __restrict__ w, const double* hcol,
-                                                       int ldh, Fin fin, Gate g, RedArgs ra, StatArgs st) {
+                                                       int ldh, Gate g, RedArgs ra, StatArgs st) {
   if (g.off()) return;
   st.count();
   extern __shared__ double sh[];
@@ -372,10 +486,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
     w[i] = tw;
     acc[0] += (double)tw * (double)tw;
   }
-  __shared__ double sums[1];
-  if (grid_sum<1>(acc, ra, sums)) {
-    if (threadIdx.x == 0) fin(sums);
-  }
+  block_store<1>(acc, ra);
 }
 
 // x += sum_{i<size} y_i B_i   (or only the last term with KS_COMPAT_LAST_TERM

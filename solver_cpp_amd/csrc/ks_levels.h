@@ -533,34 +533,48 @@ struct BicgLevel : Level {
 
 // --------------------------------------------------- projection helper --
 // h[i] = (w, V_i), i <= k, in launches of up to 32 columns.
+template <class T, int GV>
+void proj_launch(ks_ctx* c, int64_t n, const T* Vi, int64_t ldv, int m, const T* w, double* hi, int ldh, Gate g) {
+  const int G = grid_map(n);
+  const double bytes = (double)sizeof(T) * n * (m + 1);
+  c->prof.begin(c->stream, KS_K_CGS);
+  k_proj<T, GV><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
+  c->prof.end(c->stream);
+  KS_HIPCK(hipGetLastError());
+  launch_finish<GV>(c, G, FinProj{hi, ldh, m}, g);
+}
+
 template <class T>
 void launch_proj(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T* w, double* hcol, int ldh,
                  Gate g) {
   for (int i0 = 0; i0 < nv; i0 += 32) {
     const int m = std::min(32, nv - i0);
-    const int G = grid_map(n);
-    const double bytes = (double)sizeof(T) * n * (m + 1);
     const T* Vi = V + (int64_t)i0 * ldv;
     double* hi = hcol + (int64_t)i0 * ldh;
-    c->prof.begin(c->stream, KS_K_CGS);
-    if (m <= 4) k_proj<T, 4><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
-    else if (m <= 8) k_proj<T, 8><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
-    else if (m <= 16) k_proj<T, 16><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
-    else k_proj<T, 32><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, hi, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
-    c->prof.end(c->stream);
-    KS_HIPCK(hipGetLastError());
+    if (m <= 4) proj_launch<T, 4>(c, n, Vi, ldv, m, w, hi, ldh, g);
+    else if (m <= 8) proj_launch<T, 8>(c, n, Vi, ldv, m, w, hi, ldh, g);
+    else if (m <= 16) proj_launch<T, 16>(c, n, Vi, ldv, m, w, hi, ldh, g);
+    else proj_launch<T, 32>(c, n, Vi, ldv, m, w, hi, ldh, g);
   }
 }
+
+template <class Fin>
+struct FinWrap {
+  Fin f;
+  __device__ void fin(const double* s) const { f(s); }
+};
 
 template <class T, class Fin>
 void launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, T* w, const double* hcol, int ldh,
                        Fin f, Gate g) {
   const double bytes = (double)sizeof(T) * n * (nv + 2);
+  const int G = grid_map(n);
   c->prof.begin(c->stream, KS_K_CGS);
-  k_cgs_update<T, Fin><<<grid_map(n), kBlock, sizeof(double) * std::max(nv, 1), c->stream>>>(
-      n, V, ldv, nv, w, hcol, ldh, f, g, red(c), stat(c, KS_K_CGS, bytes));
+  k_cgs_update<T><<<G, kBlock, sizeof(double) * std::max(nv, 1), c->stream>>>(
+      n, V, ldv, nv, w, hcol, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
+  launch_finish<1>(c, G, FinWrap<Fin>{f}, g);
 }
 
 inline int64_t pad64(int64_t n) { return (n + 63) & ~(int64_t)63; }

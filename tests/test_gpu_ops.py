@@ -56,6 +56,24 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case):
     assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
 
 
+@pytest.mark.parametrize("case", ["poisson2d", "convdiff3d", "aniso"])
+def test_spmv_bitexact_short_rows(ctx, oracle, case):
+    # rows of <= 8 entries are summed sequentially in CSR order from products
+    # rounded in the value type: the oracle's (the reference's) exact order
+    if case == "poisson2d":
+        ptr, col, val, _, _ = P.poisson2d(41)
+    elif case == "convdiff3d":
+        ptr, col, val, _, _ = P.convdiff3d(13)
+    else:
+        ptr, col, val, _ = P.aniso(37)
+    n = len(ptr) - 1
+    x = np.random.default_rng(11).standard_normal(n)
+    A = _mat(ctx, ptr, col, val, fp32=True)
+    assert np.array_equal(A.spmv(x), oracle.spmv(ptr, col, val, x))
+    x32 = x.astype(np.float32)
+    assert np.array_equal(A.spmv(x32, precision=32), oracle.spmv_f32(ptr, col, val.astype(np.float32), x32))
+
+
 def test_spmv_edge_cases(ctx, oracle):
     # empty rows, a dense row, a single entry
     n = 70
@@ -76,6 +94,23 @@ def test_spmv_edge_cases(ctx, oracle):
     tp, tc, tv = A.transpose_csr()
     otp, otc, otv = oracle.transpose(ptr, col, val)
     assert np.array_equal(tp, otp) and np.array_equal(tc, otc) and np.array_equal(tv, otv)
+
+
+def test_spmv_long_rows(ctx, oracle):
+    # Zipf rows up to 4096 entries: rows longer than one LDS tile (2048)
+    ptr, col, val, _ = P.zipf_small(20_000, 4096)
+    assert np.diff(ptr).max() > 2048
+    n = len(ptr) - 1
+    x = np.random.default_rng(9).standard_normal(n)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    m = np.diff(ptr).max()
+    y = A.spmv(x)
+    yo = oracle.spmv(ptr, col, val, x)
+    assert np.all(np.abs(y - yo) <= 4 * m * EPS64 * P.row_abs_bound(ptr, col, val, x) + 1e-300)
+    yt = A.spmv(x, trans=True)
+    otp, otc, otv = oracle.transpose(ptr, col, val)
+    yto = oracle.spmv(otp, otc, otv, x)
+    assert np.all(np.abs(yt - yto) <= 4 * np.diff(otp).max() * EPS64 * P.row_abs_bound(otp, otc, otv, x) + 1e-300)
 
 
 def test_bad_csr_rejected(ctx):

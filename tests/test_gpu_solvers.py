@@ -139,7 +139,7 @@ def test_vpcg_fletcher_reeves_compat(ctx, oracle):
 def test_zero_rhs(ctx):
     from solver_cpp_amd import api
     ptr, col, val, b, xt = P.poisson2d(8)
-    A = _mat(ctx, ptr, col, val)
+    A = _mat(ctx, ptr, col, val, transpose=True)
     for name in ("cg", "gmres", "bicg"):
         x, r, h = api.solve(ctx, A, np.zeros(len(b)), name, outer_restart=10)
         assert r["status"] == 0 and np.all(x == 0)
