@@ -36,6 +36,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GMRES comparison")
     ap.add_argument("--cpu-steps", type=int, default=2, help="outer steps in the CPU sample")
+    ap.add_argument("--suite", action="store_true",
+                    help="also run the per-configuration SpMV, parity-solve and adjoint measurements "
+                         "(one JSON line each, after the headline line)")
+    ap.add_argument("--fd-dirs", type=int, default=64, help="finite-difference directions per adjoint case")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="MiB of one plain device copy at the end (counter calibration)")
     return ap.parse_args()
 
 
@@ -135,7 +141,136 @@ def main():
                                "kind": "port",
                                "sample": f"oracle VPGMRES (FP32 inner) {rc['steps']} outer steps timed "
                                          f"({tc:.2f} s), scaled to the GPU's {r0['steps']} outer steps"}
-    print(json.dumps(out))
+    if a.calib_copy:
+        src = torch.ones(a.calib_copy * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = 2 * src.numel() * 8
+    print(json.dumps(out), flush=True)
+    if a.suite:
+        A.close()
+        suite(ctx, a)
+
+
+def spmv_gbps(A, n, nnz, prec, trans=False, reps=30):
+    import torch
+    dt = torch.float64 if prec == 64 else torch.float32
+    xx = torch.randn(n, dtype=dt, device="cuda")
+    yy = torch.empty_like(xx)
+    A.spmv_bench(xx, yy, reps=3, trans=trans, precision=prec)
+    ms = A.spmv_bench(xx, yy, reps=reps, trans=trans, precision=prec)
+    sv = prec // 8
+    by = (sv + 4) * nnz + 4 * (n + 1) + 2 * sv * n
+    return {"us": round(ms * 1e3, 2), "GBps": round(by / (ms * 1e-3) / 1e9, 1),
+            "frac_of_8TBps": round(by / (ms * 1e-3) / 8e12, 4)}
+
+
+def suite(ctx, a):
+    """Per-configuration measurements: SpMV bandwidth (configs 2, 3, 5a, 5b),
+    the configuration-1/2 solves, and the adjoint with its FD check
+    (configurations 4 and 5)."""
+    import numpy as np
+    import torch
+    from solver_cpp_amd import api, gen
+    from solver_cpp_amd import autograd as AG
+
+    def emit(d):
+        print(json.dumps(d), flush=True)
+
+    # --- SpMV bandwidth and parity solves ------------------------------
+    for cfg in ("poisson2d", "convdiff3d", "poisson3d_sym", "poisson3d", "convdiff3d_pe08", "zipf"):
+        P = gen.problem(cfg)
+        n, nnz = len(P["ptr"]) - 1, len(P["col"])
+        nonsym = cfg in ("convdiff3d", "convdiff3d_pe08", "zipf")
+        A = api.Matrix(ctx, P["ptr"], P["col"], P["val"], fp32=True, transpose=nonsym)
+        rec = {"suite": "spmv", "config": cfg, "n": n, "nnz": nnz,
+               "fp64": spmv_gbps(A, n, nnz, 64), "fp32": spmv_gbps(A, n, nnz, 32)}
+        if nonsym:
+            rec["fp64_transpose"] = spmv_gbps(A, n, nnz, 64, trans=True)
+        emit(rec)
+        b = torch.from_numpy(P["b"]).cuda()
+        runs = []
+        if cfg == "poisson2d":
+            runs = [("cg", "NO", dict(outer_eps=1e-12, outer_maxloop=5000))]
+        elif cfg == "convdiff3d":
+            runs = [("bicg", "NO", dict(outer_eps=1e-10, outer_maxloop=20000)),
+                    ("gmres", "NO", dict(outer_eps=1e-10, outer_maxloop=20000, outer_restart=31))]
+        elif cfg == "poisson3d_sym":
+            runs = [("kskipcg", "NO", dict(outer_eps=1e-10, outer_maxloop=20000, outer_kskip=4)),
+                    ("cg", "NO", dict(outer_eps=1e-10, outer_maxloop=20000))]
+        for outer, inner, kw in runs:
+            api.solve(ctx, A, b, outer, inner, history=False, **kw)  # warm-up
+            x, r, _ = api.solve(ctx, A, b, outer, inner, history=False, **kw)
+            rec = {"suite": "solve", "config": cfg, "solver": outer, "status": r["status"], "iters": r["iters"],
+                   "seconds": round(r["seconds"], 5), "relres_true": r["relres_true"],
+                   "GBps": round(r["bytes"] / r["seconds"] / 1e9, 1)}
+            if P.get("x_true") is not None:
+                xt = torch.from_numpy(P["x_true"]).cuda()
+                rec["err_vs_x_true"] = float(torch.linalg.norm(x - xt) / torch.linalg.norm(xt))
+            emit(rec)
+        A.close()
+
+    # --- adjoint: configuration 4 (SPD) and 5 (nonsymmetric) -------------
+    rng_dirs = {"aniso2d": 42, "convdiff3d_pe08": 54, "zipf": 54}
+    for cfg in ("aniso2d", "convdiff3d_pe08", "zipf"):
+        P = gen.problem(cfg)
+        n, nnz = len(P["ptr"]) - 1, len(P["col"])
+        spd = cfg == "aniso2d"
+        b = torch.from_numpy(P["b"]).cuda()
+        xt = torch.from_numpy(P["x_target"]).cuda()
+        if spd:
+            # forward and adjoint to 1e-14: the FD comparison at 1e-6 needs
+            # gradients accurate well below 1e-6 relative
+            fwd = dict(outer_eps=1e-14, outer_maxloop=20000, inner_maxloop=10, inner_precision=32)
+            sysm = AG.SparseSystem(ctx, P["ptr"], P["col"], spd=True, solver="vpcg", inner="cg", fp32=True,
+                                   params=fwd, adjoint_solver="vpcg", adjoint_inner="cg",
+                                   adjoint_params=dict(fwd))
+            fd_solver, fd_inner = "vpcg", "cg"
+            fd_params = dict(outer_eps=1e-14, outer_maxloop=20000, inner_maxloop=10, inner_precision=32)
+        else:
+            fwd = dict(outer_eps=1e-14, outer_maxloop=20000, outer_restart=21, inner_maxloop=11,
+                       inner_restart=12, inner_precision=32)
+            sysm = AG.SparseSystem(ctx, P["ptr"], P["col"], spd=False, solver="vpgmres", inner="gmres",
+                                   fp32=True, params=fwd, adjoint_solver="vpgmres", adjoint_inner="gmres",
+                                   adjoint_params=dict(fwd))
+            fd_solver, fd_inner = "vpgmres", "gmres"
+            fd_params = dict(fwd)
+        vals = torch.from_numpy(P["val"]).cuda().requires_grad_(True)
+        bb = b.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = sysm.solve(vals, bb)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L = 0.5 * ((x - xt) ** 2).sum()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        rec = {"suite": "adjoint", "config": cfg, "n": n, "nnz": nnz, "forward_s": round(t1 - t0, 4),
+               "backward_s": round(t2 - t1, 4), "forward_iters": sysm.last_forward["iters"],
+               "adjoint_iters": sysm.last_backward["iters"],
+               "adjoint_relres": sysm.last_backward["relres_true"]}
+        # central FD along seeded directions (symmetric edge perturbations for the SPD case)
+        rng = np.random.default_rng(rng_dirs[cfg])
+        errs = []
+        gv = vals.grad
+        for _ in range(a.fd_dirs):
+            if spd:
+                dk = rng.standard_normal(len(P["kappa"]))
+                dv = torch.from_numpy(gen.aniso2d_values(P["grid"], P["grid"], dk)).cuda()
+            else:
+                dv = torch.from_numpy(rng.standard_normal(nnz)).cuda()
+            fd = AG.central_fd(ctx, P["ptr"], P["col"], vals.detach(), dv, b, xt, 1e-6, fd_solver, fd_inner,
+                               params=fd_params, spd=spd, naive=False)
+            ad = float(gv @ dv)
+            errs.append(abs(fd - ad) / abs(ad))
+        rec["fd_dirs"] = a.fd_dirs
+        rec["fd_max_rel_err"] = max(errs) if errs else None
+        rec["fd_median_rel_err"] = float(np.median(errs)) if errs else None
+        emit(rec)
+        sysm.A.close()
 
 
 if __name__ == "__main__":

@@ -143,6 +143,7 @@ struct Plan {
   int L = 8;
   int64_t ntiles = 0;
   DBuf<int32_t> tiles;  // tile row starts, ntiles + 1
+  DBuf<int32_t> tnz;    // tile entry starts (ptr[tiles[t]]), ntiles + 1
 };
 
 template <class T>
@@ -155,6 +156,7 @@ struct MatView {
   int L = 8;
   int64_t ntiles = 0;
   const int32_t* tiles = nullptr;
+  const int32_t* tnz = nullptr;
 };
 
 }  // namespace ks
@@ -184,7 +186,7 @@ inline ks::MatView<double> ks_mat::view<double>(bool trans) const {
   const ks::Plan& pl = trans ? tplan : plan;
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val.p; }
-  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p;
+  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
   return v;
 }
 template <>
@@ -194,7 +196,7 @@ inline ks::MatView<float> ks_mat::view<float>(bool trans) const {
   const ks::Plan& pl = trans ? tplan : plan;
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval32.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val32.p; }
-  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p;
+  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
   return v;
 }
 
@@ -267,8 +269,8 @@ int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const 
                 double bytes) {
   const int G = grid_rows(A.ntiles);
   c->prof.begin(c->stream, cls);
-  k_spmv_stream<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.ptr, A.col, A.val, x0, x1,
-                                                             epi, g, red(c), stat(c, cls, bytes));
+  k_spmv_stream<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val,
+                                                             x0, x1, epi, g, red(c), stat(c, cls, bytes));
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
   return G;
@@ -329,9 +331,13 @@ inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t
   while (pl.L < 64 && pl.L < mean) pl.L *= 2;
   pl.kind = vector ? 1 : 0;
   std::vector<int32_t> t = build_tiles(hp, n);
+  std::vector<int32_t> z(t.size());
+  for (size_t i = 0; i < t.size(); ++i) z[i] = hp[t[i]];
   pl.ntiles = (int64_t)t.size() - 1;
   pl.tiles.alloc(t.size());
+  pl.tnz.alloc(t.size());
   KS_HIPCK(hipMemcpyAsync(pl.tiles.p, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice, c->stream));
+  KS_HIPCK(hipMemcpyAsync(pl.tnz.p, z.data(), sizeof(int32_t) * z.size(), hipMemcpyHostToDevice, c->stream));
   KS_HIPCK(hipStreamSynchronize(c->stream));
 }
 

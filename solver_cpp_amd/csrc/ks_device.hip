@@ -207,6 +207,16 @@ std::unique_ptr<Level> make_plain(Name nm, ks_ctx* c, const MatView<T>& A, const
       L->setup(c, A, cfg);
       return L;
     }
+    case CR: {
+      auto L = std::make_unique<CrLevel<T>>();
+      L->setup(c, A, cfg);
+      return L;
+    }
+    case GCR: {
+      auto L = std::make_unique<GcrLevel<T>>();
+      L->setup(c, A, cfg);
+      return L;
+    }
     default: break;
   }
   if constexpr (sizeof(T) == 8) {
@@ -244,7 +254,8 @@ std::unique_ptr<Level> make_solver(ks_ctx* c, ks_mat* m, const ks_params* p, boo
   auto AT64 = m->has_t ? m->view<double>(!useT) : A64;
   const LevelCfg oc = outer_cfg(p, hist_cap);
   if (!vp) return make_plain<double>(on, c, A64, AT64, oc);
-  KS_CHECK(on == VPCG || on == VPGMRES, KS_ERR_UNSUPPORTED, "VP outer solver not supported");
+  KS_CHECK(on == VPCG || on == VPGMRES || on == VPGCR, KS_ERR_UNSUPPORTED,
+           "VPCR is not implemented (the reference marks it broken, README.md:15,32)");
   const LevelCfg ic = inner_cfg(p);
   std::unique_ptr<Level> inner;
   if (p->inner_precision == KS_F32) {
@@ -259,6 +270,11 @@ std::unique_ptr<Level> make_solver(ks_ctx* c, ks_mat* m, const ks_params* p, boo
   }
   if (on == VPCG) {
     auto L = std::make_unique<VpcgLevel>();
+    L->setup(c, A64, oc, std::move(inner));
+    return L;
+  }
+  if (on == VPGCR) {
+    auto L = std::make_unique<GcrLevel<double>>();
     L->setup(c, A64, oc, std::move(inner));
     return L;
   }
@@ -324,6 +340,7 @@ int64_t total_inner_steps(Level& L) {
   Level* in = nullptr;
   if (auto* g = dynamic_cast<GmresLevel<double>*>(&L)) in = g->inner.get();
   if (auto* v = dynamic_cast<VpcgLevel*>(&L)) in = v->inner.get();
+  if (auto* q = dynamic_cast<GcrLevel<double>*>(&L)) in = q->inner.get();
   if (!in) return 0;
   Ctl h;
   KS_HIPCK(hipMemcpy(&h, in->dctl.p, sizeof(Ctl), hipMemcpyDeviceToHost));

@@ -124,28 +124,34 @@ __device__ void block_store(double (&acc)[NV], RedArgs ra) {
   }
 }
 
-// Totals of the G partials of each of the NV values, summed in a fixed order
-// (deterministic), then f.fin(sums) on one thread.
+// Totals of the G partials of each of the NV values, then f.fin(sums) on one
+// thread.  TPV = kBlock / NVP threads share one value (NVP = NV rounded up to
+// a power of two, at least 4): each sums a fixed strided subset with all
+// loads independent, then a shuffle tree inside the TPV-lane group - a fixed
+// order, so the totals are deterministic.
 template <int NV, class F>
 __global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restrict__ part, F f, Gate g) {
   if (g.off()) return;
-  __shared__ double sw[NV][kBlock / 64];
-  __shared__ double sums[NV];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  constexpr int nw = kBlock / 64;
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < G; b += kBlock) s += part[(size_t)i * kMaxGrid + b];
-    s = wave_sum(s);
-    if (lane == 0) sw[i][w] = s;
+  constexpr int NVP = NV <= 4 ? 4 : NV <= 8 ? 8 : NV <= 16 ? 16 : NV <= 32 ? 32 : 64;
+  constexpr int TPV = kBlock / NVP;  // 4..64, a divisor of the wavefront
+  __shared__ double sums[NVP];
+  const int v = threadIdx.x / TPV, j = threadIdx.x % TPV;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (v < NV) {
+    const double* p = part + (size_t)v * kMaxGrid;
+    int b = j;
+    for (; b + 3 * TPV < G; b += 4 * TPV) {
+      s0 += p[b];
+      s1 += p[b + TPV];
+      s2 += p[b + 2 * TPV];
+      s3 += p[b + 3 * TPV];
+    }
+    for (; b < G; b += TPV) s0 += p[b];
   }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double s = 0.0;
+  double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
-    for (int k = 0; k < nw; ++k) s += sw[threadIdx.x][k];
-    sums[threadIdx.x] = s;
-  }
+  for (int off = TPV / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (j == 0) sums[v] = s;
   __syncthreads();
   if (threadIdx.x == 0) f.fin(sums);
 }
@@ -244,13 +250,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
 
 // CSR-stream SpMV.  The rows are cut (on the host, once per matrix) into
 // tiles of at most kTileRows rows and kTileNnz entries; a row longer than
-// kTileNnz is a tile of its own.  Per tile the workgroup (1) streams the
-// tile's values and column indices with independent loads (eight per thread
-// in flight) and gathers x, writing the products to LDS, then (2) sums each
-// row's products from LDS with `tpr` threads per row (tpr = 1 when the tile
-// is full of short rows: a sequential sum in CSR order, the reference's
-// blas::MtxVec_mult order, blas.cpp:56-62).  NX = 2 forms A x0 and A x1 from
-// one pass over the matrix (the k-skip matrix-powers basis).
+// kTileNnz is a tile of its own.  tiles[t] / tnz[t] are the first row and
+// first entry of tile t.  Per tile the workgroup (1) multiplies the tile's
+// values by the gathered x and writes the products to LDS, then (2) sums
+// each row's products from LDS with `tpr` threads per row (tpr = 1 for rows
+// of up to 8 entries: a sequential sum in CSR order, the reference's
+// blas::MtxVec_mult order, blas.cpp:56-62).  The loop is software-pipelined:
+// while tile t is summed, the bounds, row pointers, values and column
+// indices of the workgroup's next tile are already in flight (eight
+// independent loads per thread).  NX = 2 forms A x0 and A x1 from one pass
+// over the matrix (the k-skip matrix-powers basis).
 constexpr int kTileNnz = 2048;
 constexpr int kTileRows = 256;
 constexpr int kTileUnroll = kTileNnz / kBlock;
@@ -258,8 +267,30 @@ constexpr int kTileUnroll = kTileNnz / kBlock;
 template <class T>
 __device__ inline T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
 
+template <class T>
+struct TileData {
+  T v[kTileUnroll];
+  int c[kTileUnroll];
+  int32_t pr;  // row pointer of row (r0 + tid)
+  __device__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                       const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1) {
+    const int cnt = j1 - j0;
+    if (cnt > kTileNnz) return;  // long row: loaded in place
+#pragma unroll
+    for (int q = 0; q < kTileUnroll; ++q) {
+      const int k = threadIdx.x + q * kBlock;
+      if (k < cnt) {
+        v[q] = ld_stream(val + j0 + k);
+        c[q] = ld_stream(col + j0 + k);
+      }
+    }
+    pr = (int)threadIdx.x < r1 - r0 ? ptr[r0 + threadIdx.x] : 0;
+  }
+};
+
 template <class T, int NX, int NV, class Epi>
 __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
+                                                        const int32_t* __restrict__ tnz,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ col,
                                                         const T* __restrict__ val, const T* __restrict__ x0,
@@ -278,12 +309,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntile
   RowBlocks rbs{ntiles};
   int64_t t, end, step;
   rbs.range(t, end, step);
+  int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
+  TileData<T> d;
+  if (t < end) {
+    r0 = tiles[t]; r1 = tiles[t + 1]; j0 = tnz[t]; j1 = tnz[t + 1];
+    d.load(ptr, col, val, r0, r1, j0, j1);
+  }
   for (; t < end; t += step) {
-    const int32_t r0 = tiles[t], r1 = tiles[t + 1];
-    const int nr = r1 - r0;
-    const int32_t j0 = ptr[r0];
-    const int cnt = ptr[r1] - j0;
-    __syncthreads();  // LDS of the previous tile is free
+    const int nr = r1 - r0, cnt = j1 - j0;
+    const int64_t tn = t + step;
+    const bool more = tn < end;
+    int32_t nr0 = 0, nr1 = 0, nj0 = 0, nj1 = 0;
+    if (more) { nr0 = tiles[tn]; nr1 = tiles[tn + 1]; nj0 = tnz[tn]; nj1 = tnz[tn + 1]; }
     if (cnt > kTileNnz) {
       // one long row: strided partial sums, then a fixed tree
       T s0 = 0, s1 = 0;
@@ -312,45 +349,41 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntile
         }
         epi.row(r0, a0, a1, acc);
       }
-      continue;
-    }
-    for (int q = tid; q <= nr; q += kBlock) sptr[q] = ptr[r0 + q] - j0;
-    T v[kTileUnroll];
-    int c[kTileUnroll];
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1);
+      __syncthreads();
+    } else {
+      if (tid < nr) sptr[tid] = d.pr - j0;
+      if (tid == 0) sptr[nr] = cnt;
 #pragma unroll
-    for (int q = 0; q < kTileUnroll; ++q) {
-      const int k = tid + q * kBlock;
-      if (k < cnt) {
-        v[q] = ld_stream(val + j0 + k);
-        c[q] = ld_stream(col + j0 + k);
+      for (int q = 0; q < kTileUnroll; ++q) {
+        const int k = tid + q * kBlock;
+        if (k < cnt) {
+          prod[0][k] = d.v[q] * x0[d.c[q]];
+          if (NX == 2) prod[NX - 1][k] = d.v[q] * x1[d.c[q]];
+        }
       }
+      __syncthreads();
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1);  // in flight during the sums
+      // more than one thread per row only for rows averaging > 8 entries
+      int tpr = 1;
+      while (tpr < 64 && tpr * 2 * nr <= kBlock && tpr * 8 * nr < cnt) tpr *= 2;
+      const int grp = tid / tpr, lane = tid % tpr;
+      if (grp < nr) {
+        const int s = sptr[grp], e = sptr[grp + 1];
+        T a0 = 0, a1 = 0;
+        for (int k = s + lane; k < e; k += tpr) {
+          a0 += prod[0][k];
+          if (NX == 2) a1 += prod[NX - 1][k];
+        }
+        for (int off = tpr >> 1; off > 0; off >>= 1) {
+          a0 += __shfl_xor(a0, off, 64);
+          if (NX == 2) a1 += __shfl_xor(a1, off, 64);
+        }
+        if (lane == 0) epi.row(r0 + grp, a0, a1, acc);
+      }
+      __syncthreads();
     }
-#pragma unroll
-    for (int q = 0; q < kTileUnroll; ++q) {
-      const int k = tid + q * kBlock;
-      if (k < cnt) {
-        prod[0][k] = v[q] * x0[c[q]];
-        if (NX == 2) prod[NX - 1][k] = v[q] * x1[c[q]];
-      }
-    }
-    __syncthreads();
-    // more than one thread per row only for rows averaging > 8 entries
-    int tpr = 1;
-    while (tpr < 64 && tpr * 2 * nr <= kBlock && tpr * 8 * nr < cnt) tpr *= 2;
-    const int grp = tid / tpr, lane = tid % tpr;
-    if (grp < nr) {
-      const int s = sptr[grp], e = sptr[grp + 1];
-      T a0 = 0, a1 = 0;
-      for (int k = s + lane; k < e; k += tpr) {
-        a0 += prod[0][k];
-        if (NX == 2) a1 += prod[NX - 1][k];
-      }
-      for (int off = tpr >> 1; off > 0; off >>= 1) {
-        a0 += __shfl_xor(a0, off, 64);
-        if (NX == 2) a1 += __shfl_xor(a1, off, 64);
-      }
-      if (lane == 0) epi.row(r0 + grp, a0, a1, acc);
-    }
+    r0 = nr0; r1 = nr1; j0 = nj0; j1 = nj1;
   }
   if constexpr (NV > 0) block_store<NV>(acc, ra);
 }
@@ -524,6 +557,33 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __rest
   }
 }
 
+// GCR block update (blas::Gcr_sp_1 + Vec_add, blas.cpp:526-543):
+// beta_i = -(Av, q_i)/qq_i (hq = (Av, q_i) from k_proj), then
+// P_{k+1} = z + sum_i beta_i P_i and Q_{k+1} = Av + sum_i beta_i Q_i with the
+// sums accumulated in i order from 0 (the reference's Scalar_axy sequence).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gcr_update(int64_t n, const T* __restrict__ P, const T* __restrict__ Q,
+                                                       int64_t ldv, int nv, const T* __restrict__ z,
+                                                       const T* __restrict__ Av, const double* hq,
+                                                       const double* qq, T* __restrict__ Pn, T* __restrict__ Qn,
+                                                       Gate g, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  extern __shared__ double sb[];
+  for (int v = threadIdx.x; v < nv; v += kBlock) sb[v] = -hq[v] / qq[v];
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    double tp = 0.0, tq = 0.0;
+    for (int v = 0; v < nv; ++v) {
+      tp = sb[v] * (double)P[(int64_t)v * ldv + i] + tp;
+      tq = sb[v] * (double)Q[(int64_t)v * ldv + i] + tq;
+    }
+    Pn[i] = (T)((double)z[i] + (double)(T)tp);
+    Qn[i] = (T)((double)Av[i] + (double)(T)tq);
+  }
+}
+
 // ------------------------------------------------------------ GMRES ----
 // Back substitution y = H(0:sz,0:sz)^{-1} e (blas.cpp:303-314).
 __global__ void k_hye(Ctl* c, Gate g) {
@@ -545,10 +605,9 @@ __global__ void k_mark_done(Ctl* c, Gate g) {
 }
 
 // Givens rotations for column k (gmres.cpp:384-401), then the stop test of
-// step k+1 when it lies in this cycle (gmres.cpp:278-324).
-__global__ void k_givens(Ctl* c, int k, Gate g, StatArgs st) {
-  if (g.off()) return;
-  st.count();
+// step k+1 when it lies in this cycle (gmres.cpp:278-324).  Runs in the
+// finisher of the Gram-Schmidt norm, right after h_{k+1,k} is known.
+__device__ inline void givens_step(Ctl* c, int k) {
   double* H = c->H;
   const int ld = c->ld;
   for (int i = 0; i <= k - 1; ++i) {

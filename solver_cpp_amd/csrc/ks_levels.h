@@ -278,6 +278,7 @@ struct FinGmNorm {
     const double h = sqrt(s[0]);
     c->H[(k + 1) * c->ld + k] = h;
     c->vscale = h;
+    givens_step(c, k);
   }
 };
 
@@ -487,6 +488,98 @@ struct CgLevel : Level {
   }
 };
 
+// ------------------------------------------------------------- CR ------
+// cr::solve (cr.cpp:152-280): alpha = (r,s)/(q,q), s = A r,
+// beta = (r',s')/(r,s), p = beta p + r, q = beta q + s.  (q,q) is summed in
+// the pass that forms q; (r,s) in the SpMV that forms s.
+struct FinCrBeta {
+  Ctl* c;
+  __device__ void operator()(const double* s) const {
+    c->beta = s[0] / c->rz;
+    c->rz = s[0];
+  }
+};
+// after q = A p (init): s = q, acc0 (r,s), acc1 (q,q), acc2 (r,r); test at loop 1
+template <class T>
+struct OpCrInit {
+  const T *r, *q;
+  T* s;
+  Ctl* c;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) {
+    const double ri = (double)r[i], qi = (double)q[i];
+    s[i] = q[i];
+    acc[0] += ri * qi;
+    acc[1] += qi * qi;
+    acc[2] += ri * ri;
+  }
+  __device__ void fin(const double* x) {
+    c->rz = x[0];
+    c->alpha = x[0] / x[1];
+    c->count = 1;
+    cg_check(c, x[2]);
+  }
+};
+// p = beta p + r; q = beta q + s; acc (q,q) -> alpha of the next iteration
+template <class T>
+struct OpCrXpby {
+  T *p, *q;
+  const T *r, *s;
+  Ctl* c;
+  double b;
+  __device__ void prep() { b = c->beta; }
+  __device__ void elem(int64_t i, double* acc) {
+    p[i] = (T)(b * (double)p[i] + (double)r[i]);
+    const T qi = (T)(b * (double)q[i] + (double)s[i]);
+    q[i] = qi;
+    acc[0] += (double)qi * (double)qi;
+  }
+  __device__ void fin(const double* x) { c->alpha = c->rz / x[0]; }
+};
+
+template <class T>
+struct CrLevel : Level {
+  MatView<T> A;
+  DBuf<T> r, p, q, s, xown;
+  T* x = nullptr;
+  double* xout = nullptr;
+  void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf) {
+    c = cx; A = a; n = a.n; cfg = cf;
+    r.alloc(n); p.alloc(n); q.alloc(n); s.alloc(n);
+    if (sizeof(T) != 8) xown.alloc(n);
+    setup_ctl(0, 0);
+  }
+  void enqueue_init(const double* b, double* x_out) override {
+    xout = x_out;
+    x = sizeof(T) == 8 ? (T*)x_out : xown.p;
+    const double* src = b;
+    if (x0given) {
+      launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinNone>{b, r.p, FinNone{}}, gouter(), 8.0 * n);
+      src = (const double*)r.p;
+    }
+    OpCgInit<T, 2> op{src, x, r.p, p.p, nullptr, nullptr, ctl(), !x0given, !x0given};
+    launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, 32.0 * n);
+    launch_spmv<T, 0>(c, A, p.p, EpiStore<T>{q.p}, gouter());
+    launch_map<3>(c, n, OpCrInit<T>{r.p, q.p, s.p, ctl()}, gouter(), KS_K_REDUCE, 3.0 * sizeof(T) * n);
+  }
+  void enqueue_unit(int64_t) override {
+    launch_map<1>(c, n, OpCgUpdate<T, 1>{x, r.p, p.p, q.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+    launch_spmv<T, 1>(c, A, r.p, EpiDot<T, FinCrBeta>{r.p, s.p, FinCrBeta{ctl()}}, gstep(), sizeof(T) * (double)n);
+    launch_map<1>(c, n, OpCrXpby<T>{p.p, q.p, r.p, s.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+  }
+  int64_t max_units() const override { return cfg.maxloop > 0 ? cfg.maxloop : 0; }
+  void enqueue_finish() override {
+    if (sizeof(T) != 8) {
+      k_convert<T, double><<<grid_map(n), kBlock, 0, c->stream>>>(n, x, xout, gouter());
+      KS_HIPCK(hipGetLastError());
+    }
+  }
+  double unit_seconds_estimate() const override {
+    return std::max(3 * 4e-6, (spmv_bytes<T>(n, A.nnz) + 12.0 * sizeof(T) * n) / 4e12);
+  }
+};
+
 // ----------------------------------------------------------- BiCG ------
 template <class T>
 struct BicgLevel : Level {
@@ -612,7 +705,14 @@ struct GmresLevel : Level {
     if (cfg.maxloop <= 0) return 0;
     return (int64_t)(cfg.maxloop - 1) / (R - 1) + 1;
   }
-  int64_t max_units() const override { return cycles() * (R - 1); }
+  // The test of step k runs in unit k-1 (the Givens finisher) except at a
+  // cycle start, so unit maxloop-1 does no work unless it starts a cycle.
+  int64_t max_units() const override {
+    if (cfg.maxloop <= 0) return 0;
+    const int64_t last = cfg.maxloop - 1;
+    const int64_t u = last % (R - 1) == 0 ? last + 1 : last;
+    return std::min<int64_t>(u, cycles() * (R - 1));
+  }
   void enqueue_init(const double* bb, double* x_out) override {
     b = bb;
     xout = x_out;
@@ -649,8 +749,6 @@ struct GmresLevel : Level {
     launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
     launch_map<0>(c, n, OpScale<T>{w.p, V.p + (int64_t)(k + 1) * ldv, h, 0.0}, gstep(), KS_K_AXPY,
                   2.0 * sizeof(T) * n);
-    k_givens<<<1, 1, 0, c->stream>>>(h, k, gstep(), stat(c, KS_K_SCALAR, 0));
-    KS_HIPCK(hipGetLastError());
     if (k == R - 2) cycle_end();
   }
   void cycle_end() {
@@ -681,6 +779,172 @@ struct GmresLevel : Level {
   int64_t reported_iters(const Ctl& hc) const override { return hc.count + 1; }
   double unit_seconds_estimate() const override {
     double s = (spmv_bytes<T>(n, A.nnz) + (double)sizeof(T) * n * (R + 8)) / 4e12 + 5 * 4e-6;
+    if (inner) s += inner->max_units() * inner->unit_seconds_estimate();
+    return s;
+  }
+};
+
+// ------------------------------------------------------------ GCR ------
+// gcr::solve (gcr.cpp:178-313); with `inner` it is vpgcr::solve
+// (vpgcr.cpp:693-860): P_0 = M(r) and each new direction starts from
+// z = M(r).  loop starts at 1 (gcr.cpp:41); the test order is eps first,
+// then the iteration limit (gcr.cpp:226-234).  Ctl arrays: e = qq, y = (Av,q_i).
+template <class T>
+struct OpGcrDots {  // acc0 (q,q), acc1 (r,q), acc2 (r,r)
+  const T *q, *r;
+  Ctl* c;
+  int k;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double* acc) {
+    const double qi = (double)q[i], ri = (double)r[i];
+    acc[0] += qi * qi;
+    acc[1] += ri * qi;
+    acc[2] += ri * ri;
+  }
+  __device__ void fin(const double* s) {
+    const double err = relres(c, sqrt(s[2]));
+    ctl_hist(c, (double)c->count, err);
+    if (err <= c->eps) { ctl_stop(c, 0); return; }
+    if (!finite(err)) { ctl_stop(c, 3); return; }
+    if (c->count >= c->maxloop) { ctl_stop(c, 2); return; }
+    c->count++;
+    c->steps++;
+    c->tsteps++;
+    c->e[k] = s[0];
+    c->alpha = s[1] / s[0];
+  }
+};
+template <class T>
+struct OpGcrX {  // x += a P_k; r -= a Q_k (unless the cycle's last step)
+  T *x, *r;
+  const T *pk, *qk;
+  const Ctl* c;
+  int upd_r;
+  double a;
+  __device__ void prep() { a = c->alpha; }
+  __device__ void elem(int64_t i, double*) {
+    x[i] = (T)(a * (double)pk[i] + (double)x[i]);
+    if (upd_r) r[i] = (T)((-a) * (double)qk[i] + (double)r[i]);
+  }
+  __device__ void fin(const double*) {}
+};
+// cycle start after the first: the while (loop < maxloop) test (gcr.cpp:188)
+struct OpGcrWhile {
+  Ctl* c;
+  __device__ void run() {
+    if (c->count >= c->maxloop) ctl_stop(c, 2);
+  }
+};
+template <class T>
+struct OpCopy {
+  const T* a;
+  T* b;
+  __device__ void prep() {}
+  __device__ void elem(int64_t i, double*) { b[i] = a[i]; }
+  __device__ void fin(const double*) {}
+};
+
+template <class T>
+struct GcrLevel : Level {
+  MatView<T> A;
+  int R = 0;
+  int64_t ldv = 0;
+  DBuf<T> P, Q, r, Av, z, xown;
+  T* x = nullptr;
+  double* xout = nullptr;
+  const double* b = nullptr;
+  std::unique_ptr<Level> inner;  // VPGCR (T = double)
+  void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf, std::unique_ptr<Level> in = nullptr) {
+    c = cx; A = a; n = a.n; cfg = cf; R = cf.restart;
+    KS_CHECK(R >= 1, KS_ERR_BAD_INPUT, "GCR restart (-R/-r) must be >= 1");
+    ldv = pad64(n);
+    P.alloc((size_t)ldv * R);
+    Q.alloc((size_t)ldv * R);
+    r.alloc(n); Av.alloc(n);
+    if (sizeof(T) != 8) xown.alloc(n);
+    setup_ctl(R, 0);
+    inner = std::move(in);
+    if (inner) {
+      z.alloc(n);
+      inner->outer_stop = &dctl.p->stop;
+    }
+  }
+  int64_t cycles() const {
+    if (cfg.maxloop <= 1) return 1;
+    return (int64_t)(cfg.maxloop - 1 + R - 1) / R + 1;
+  }
+  int64_t max_units() const override { return cycles() * R; }
+  void run_inner(const double* v, double* out) {
+    inner->enqueue_init(v, out);
+    for (int64_t u = 0; u < inner->max_units(); ++u) inner->enqueue_unit(u);
+    inner->enqueue_finish();
+  }
+  void enqueue_init(const double* bb, double* x_out) override {
+    b = bb;
+    xout = x_out;
+    x = sizeof(T) == 8 ? (T*)x_out : xown.p;
+    if (x0given) launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+    else {
+      // x = 0, r = b, bnorm = ||b||; count (loop) = 1
+      OpCgInit<T, 2> op{b, x, r.p, nullptr, nullptr, nullptr, ctl(), 1, 1};
+      launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (8.0 + 2.0 * sizeof(T)) * n);
+    }
+    launch_scalar(c, OpSetCount{ctl(), 1}, gouter());
+  }
+  struct OpSetCount {
+    Ctl* c;
+    long long v;
+    __device__ void run() { c->count = v; }
+  };
+  void cycle_start(int64_t cyc) {
+    const Gate g = gstep();
+    launch_scalar(c, OpGcrWhile{ctl()}, g);
+    if (cyc > 0 || x0given)
+      launch_spmv<T, 0>(c, A, x, EpiResid<T, FinNone>{b, r.p, FinNone{}}, g, 8.0 * n);
+    T* p0 = P.p;
+    if (inner) {
+      run_inner((const double*)r.p, (double*)z.p);
+      launch_map<0>(c, n, OpCopy<T>{z.p, p0}, g, KS_K_AXPY, 2.0 * sizeof(T) * n);
+    } else {
+      launch_map<0>(c, n, OpCopy<T>{r.p, p0}, g, KS_K_AXPY, 2.0 * sizeof(T) * n);
+    }
+    launch_spmv<T, 0>(c, A, p0, EpiStore<T>{Q.p}, g);
+  }
+  void enqueue_unit(int64_t u) override {
+    const int64_t cyc = u / R;
+    const int k = (int)(u % R);
+    if (k == 0) cycle_start(cyc);
+    const Gate g = gstep();
+    T* pk = P.p + (int64_t)k * ldv;
+    T* qk = Q.p + (int64_t)k * ldv;
+    launch_map<3>(c, n, OpGcrDots<T>{qk, r.p, ctl(), k}, g, KS_K_REDUCE, 2.0 * sizeof(T) * n);
+    const int last = k == R - 1;
+    launch_map<0>(c, n, OpGcrX<T>{x, r.p, pk, qk, ctl(), !last, 0.0}, g, KS_K_AXPY,
+                  (last ? 3.0 : 6.0) * sizeof(T) * n);
+    if (last) return;
+    const T* src = r.p;
+    if (inner) {
+      run_inner((const double*)r.p, (double*)z.p);
+      src = z.p;
+    }
+    launch_spmv<T, 0>(c, A, src, EpiStore<T>{Av.p}, g);
+    launch_proj<T>(c, n, Q.p, ldv, k + 1, Av.p, hctl.y, 1, g);
+    const double bytes = (double)sizeof(T) * n * (2.0 * (k + 1) + 4.0);
+    c->prof.begin(c->stream, KS_K_CGS);
+    k_gcr_update<T><<<grid_map(n), kBlock, sizeof(double) * (k + 1), c->stream>>>(
+        n, P.p, Q.p, ldv, k + 1, src, Av.p, hctl.y, hctl.e, P.p + (int64_t)(k + 1) * ldv,
+        Q.p + (int64_t)(k + 1) * ldv, g, stat(c, KS_K_CGS, bytes));
+    c->prof.end(c->stream);
+    KS_HIPCK(hipGetLastError());
+  }
+  void enqueue_finish() override {
+    if (sizeof(T) != 8) {
+      k_convert<T, double><<<grid_map(n), kBlock, 0, c->stream>>>(n, x, xout, gouter());
+      KS_HIPCK(hipGetLastError());
+    }
+  }
+  double unit_seconds_estimate() const override {
+    double s = (spmv_bytes<T>(n, A.nnz) + (double)sizeof(T) * n * (2 * R + 8)) / 4e12 + 7 * 4e-6;
     if (inner) s += inner->max_units() * inner->unit_seconds_estimate();
     return s;
   }

@@ -1,0 +1,68 @@
+"""Summarise rocprofv3 output databases (rocpd SQLite) as short text.
+
+  python -m solver_cpp_amd.rocpd_summary trace <results.db> [--top N]
+  python -m solver_cpp_amd.rocpd_summary pmc <results.db> [--kernel SUBSTR]
+
+`trace` lists kernels by total time (calls, total and mean duration in us;
+dispatches faster than --active-us are also reported separately: those are
+launches whose device-side stop flag made them return at once).  `pmc`
+lists the mean counter value per dispatch for each kernel.
+"""
+import argparse
+import collections
+import sqlite3
+import statistics
+
+
+def short(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def trace(db, top=20, active_us=10.0, match=None):
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, duration from kernels").fetchall()
+    agg = collections.defaultdict(list)
+    for name, dur in rows:
+        agg[short(name)].append(dur / 1000.0)
+    tot = sum(sum(v) for v in agg.values())
+    out = [f"{'kernel':70s} {'calls':>6s} {'total_ms':>9s} {'mean_us':>8s} {'%':>5s} "
+           f"{'active':>6s} {'active_mean_us':>14s}"]
+    items = sorted(agg.items(), key=lambda kv: -sum(kv[1]))
+    if match:
+        items = [kv for kv in items if match in kv[0]]
+    for k, v in items[:top]:
+        act = [d for d in v if d >= active_us]
+        am = statistics.mean(act) if act else 0.0
+        out.append(f"{k[:70]:70s} {len(v):6d} {sum(v) / 1e3:9.2f} {statistics.mean(v):8.2f} "
+                   f"{100 * sum(v) / tot:5.1f} {len(act):6d} {am:14.2f}")
+    out.append(f"total kernel time {tot / 1e3:.2f} ms over {len(rows)} dispatches")
+    return "\n".join(out)
+
+
+def pmc(db, match=None):
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    agg = collections.defaultdict(list)
+    for k, c, v in rows:
+        agg[(short(k), c)].append(v)
+    out = [f"{'kernel':70s} {'counter':12s} {'dispatches':>10s} {'median_KiB':>12s} {'max_KiB':>12s}"]
+    for (k, c), vs in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        if match and match not in k:
+            continue
+        out.append(f"{k[:70]:70s} {c:12s} {len(vs):10d} {statistics.median(vs):12.1f} {max(vs):12.1f}")
+    return "\n".join(out)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["trace", "pmc"])
+    ap.add_argument("db")
+    ap.add_argument("--top", type=int, default=20)
+    ap.add_argument("--active-us", type=float, default=10.0)
+    ap.add_argument("--kernel", default=None)
+    a = ap.parse_args()
+    print(trace(a.db, a.top, a.active_us, a.kernel) if a.mode == "trace" else pmc(a.db, a.kernel))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,84 @@
+"""Adjoint (torch.autograd) checks on the GPU: against the FP64 dense
+reference gradient and against central finite differences (FP64, eps 1e-6).
+
+Tolerance: the adjoint gradient must agree with central FD to <= 1e-6
+relative in every direction (FD truncation is O(eps^2) = 1e-12 relative; the
+FP64 solves run to 1e-14)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense_grad(ptr, col, val, x, g):
+    from tests._problems import dense
+    D = dense(ptr, col, val)
+    lam = np.linalg.solve(D.T, g)
+    rows = np.repeat(np.arange(len(ptr) - 1), np.diff(ptr))
+    return lam, -lam[rows] * x[col]
+
+
+def test_spd_vpcg_forward_cg_adjoint(ctx):
+    import torch
+    from solver_cpp_amd import autograd as AG
+    from solver_cpp_amd import gen
+    g = 32
+    ptr, col, val, kap = gen.aniso2d(g, g, 4)
+    n = g * g
+    b = torch.from_numpy(gen.vector(41, n, "uniform")).cuda()
+    xt = torch.from_numpy(gen.vector(40, n, "normal")).cuda()
+    sysm = AG.SparseSystem(ctx, ptr, col, spd=True, solver="vpcg", inner="cg", fp32=True,
+                           params=dict(outer_eps=1e-14, outer_maxloop=3000, inner_maxloop=10, inner_precision=32),
+                           adjoint_solver="cg", adjoint_params=dict(outer_eps=1e-14, outer_maxloop=5000))
+    vals = torch.from_numpy(val).cuda().requires_grad_(True)
+    bb = b.clone().requires_grad_(True)
+    x = sysm.solve(vals, bb)
+    L = 0.5 * ((x - xt) ** 2).sum()
+    L.backward()
+    assert sysm.last_forward["status"] == 0 and sysm.last_backward["status"] == 0
+    lam, gv = _dense_grad(ptr, col, val, x.detach().cpu().numpy(), (x - xt).detach().cpu().numpy())
+    assert np.linalg.norm(bb.grad.cpu().numpy() - lam) <= 1e-9 * np.linalg.norm(lam)
+    assert np.linalg.norm(vals.grad.cpu().numpy() - gv) <= 1e-9 * np.linalg.norm(gv)
+    # FD along symmetric edge directions (keeps A SPD): d_val = A(d_kappa)
+    rng = np.random.default_rng(42)
+    for _ in range(6):
+        dk = rng.standard_normal(len(kap))
+        dv = torch.from_numpy(gen.aniso2d_values(g, g, dk)).cuda()
+        fd, naive = AG.central_fd(ctx, ptr, col, vals.detach(), dv, b, xt, 1e-6, "cg",
+                                  params=dict(outer_eps=1e-14, outer_maxloop=5000), spd=True)
+        ad = float(vals.grad @ dv)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad, naive)
+
+
+@pytest.mark.parametrize("case", ["convdiff", "zipf"])
+def test_nonsymmetric_vpgmres_forward_gmres_adjoint(ctx, case):
+    import torch
+    from solver_cpp_amd import autograd as AG
+    from solver_cpp_amd import gen
+    if case == "convdiff":
+        ptr, col, val = gen.stencil(3, 14, 14, 14, pe=0.8)
+    else:
+        ptr, col, val = gen.zipf(3000, 1.5, 300, (51, 52, 53))
+    n = len(ptr) - 1
+    b = torch.from_numpy(gen.vector(50, n, "uniform")).cuda()
+    xt = torch.from_numpy(gen.vector(5, n, "normal")).cuda()
+    fwd = dict(outer_eps=1e-14, outer_maxloop=2000, outer_restart=21, inner_maxloop=11, inner_restart=12,
+               inner_precision=32)
+    sysm = AG.SparseSystem(ctx, ptr, col, spd=False, solver="vpgmres", inner="gmres", fp32=True, params=fwd,
+                           adjoint_solver="gmres", adjoint_inner="NO",
+                           adjoint_params=dict(outer_eps=1e-14, outer_maxloop=20000, outer_restart=41))
+    vals = torch.from_numpy(val).cuda().requires_grad_(True)
+    bb = b.clone().requires_grad_(True)
+    x = sysm.solve(vals, bb)
+    L = 0.5 * ((x - xt) ** 2).sum()
+    L.backward()
+    lam, gv = _dense_grad(ptr, col, val, x.detach().cpu().numpy(), (x - xt).detach().cpu().numpy())
+    assert np.linalg.norm(bb.grad.cpu().numpy() - lam) <= 1e-9 * np.linalg.norm(lam)
+    assert np.linalg.norm(vals.grad.cpu().numpy() - gv) <= 1e-9 * np.linalg.norm(gv)
+    rng = np.random.default_rng(54)
+    for _ in range(6):
+        dv = torch.from_numpy(rng.standard_normal(len(val))).cuda()
+        fd, naive = AG.central_fd(ctx, ptr, col, vals.detach(), dv, b, xt, 1e-6, "vpgmres", "gmres",
+                                  params=fwd)
+        ad = float(vals.grad @ dv)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad, naive)

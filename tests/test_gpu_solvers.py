@@ -136,6 +136,39 @@ def test_vpcg_fletcher_reeves_compat(ctx, oracle):
     assert np.allclose(h[:10, 1], ho[:10, 1], rtol=1e-6)
 
 
+def test_cr(ctx, oracle):
+    ptr, col, val, b, xt = P.convdiff3d(12, pe=0.0)
+    _run(ctx, oracle, ptr, col, val, b, "cr", outer_eps=1e-10, outer_maxloop=2000)
+
+
+@pytest.mark.parametrize("R", [4, 30])
+def test_gcr(ctx, oracle, R):
+    ptr, col, val, b, xt = P.convdiff3d(11)
+    _run(ctx, oracle, ptr, col, val, b, "gcr", outer_eps=1e-10, outer_maxloop=2000, outer_restart=R)
+
+
+def test_gcr_maxloop(ctx, oracle):
+    ptr, col, val, b, xt = P.convdiff3d(8)
+    x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "gcr", outer_eps=1e-15, outer_maxloop=10,
+                               outer_restart=4)
+    assert r["status"] == 2 and r["iters"] == ro["iters"] == 10
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_vpgcr(ctx, oracle, prec):
+    ptr, col, val, b, xt = P.convdiff3d(12)
+    _run(ctx, oracle, ptr, col, val, b, "vpgcr", "gmres", outer_eps=1e-10, outer_maxloop=300, outer_restart=20,
+         inner_maxloop=6, inner_restart=7, inner_precision=prec, xtol=1e-7)
+
+
+def test_inner_solver_kinds(ctx, oracle):
+    # every inner solver the reference dispatches (innerMethods.cpp:74-111)
+    ptr, col, val, b, xt = P.convdiff3d(10, pe=0.0)
+    for inner in ("cg", "cr", "gcr", "bicg", "gmres"):
+        _run(ctx, oracle, ptr, col, val, b, "vpgmres", inner, outer_eps=1e-10, outer_maxloop=200,
+             outer_restart=21, inner_maxloop=5, inner_restart=8, xtol=1e-7)
+
+
 def test_zero_rhs(ctx):
     from solver_cpp_amd import api
     ptr, col, val, b, xt = P.poisson2d(8)
