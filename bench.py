@@ -213,8 +213,8 @@ def suite(ctx, a):
         A.close()
 
     # --- adjoint: configuration 4 (SPD) and 5 (nonsymmetric) -------------
-    rng_dirs = {"aniso2d": 42, "convdiff3d_pe08": 54, "zipf": 54}
-    for cfg in ("aniso2d", "convdiff3d_pe08", "zipf"):
+    rng_dirs = {"aniso2d": 42, "convdiff3d_pe08": 54}
+    for cfg in ("aniso2d", "convdiff3d_pe08"):
         P = gen.problem(cfg)
         n, nnz = len(P["ptr"]) - 1, len(P["col"])
         spd = cfg == "aniso2d"

@@ -315,12 +315,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntile
     r0 = tiles[t]; r1 = tiles[t + 1]; j0 = tnz[t]; j1 = tnz[t + 1];
     d.load(ptr, col, val, r0, r1, j0, j1);
   }
+  // meta of the next tile: loaded one tile ahead, after the data prefetch,
+  // so waiting for the prefetched columns never waits for these loads
+  int32_t nr0 = 0, nr1 = 0, nj0 = 0, nj1 = 0;
+  if (t + step < end) {
+    nr0 = tiles[t + step]; nr1 = tiles[t + step + 1]; nj0 = tnz[t + step]; nj1 = tnz[t + step + 1];
+  }
   for (; t < end; t += step) {
     const int nr = r1 - r0, cnt = j1 - j0;
-    const int64_t tn = t + step;
-    const bool more = tn < end;
-    int32_t nr0 = 0, nr1 = 0, nj0 = 0, nj1 = 0;
-    if (more) { nr0 = tiles[tn]; nr1 = tiles[tn + 1]; nj0 = tnz[tn]; nj1 = tnz[tn + 1]; }
+    const int64_t tn = t + step, tnn = t + 2 * step;
+    const bool more = tn < end, more2 = tnn < end;
+    int32_t mr0 = 0, mr1 = 0, mj0 = 0, mj1 = 0;
     if (cnt > kTileNnz) {
       // one long row: strided partial sums, then a fixed tree
       T s0 = 0, s1 = 0;
@@ -350,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntile
         epi.row(r0, a0, a1, acc);
       }
       if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1);
+      if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       __syncthreads();
     } else {
       if (tid < nr) sptr[tid] = d.pr - j0;
@@ -363,7 +369,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntile
         }
       }
       __syncthreads();
-      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1);  // in flight during the sums
+      // in flight during the sums: the next tile's data, then the meta after it
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1);
+      if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       // more than one thread per row only for rows averaging > 8 entries
       int tpr = 1;
       while (tpr < 64 && tpr * 2 * nr <= kBlock && tpr * 8 * nr < cnt) tpr *= 2;
@@ -384,6 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntile
       __syncthreads();
     }
     r0 = nr0; r1 = nr1; j0 = nj0; j1 = nj1;
+    nr0 = mr0; nr1 = mr1; nj0 = mj0; nj1 = mj1;
   }
   if constexpr (NV > 0) block_store<NV>(acc, ra);
 }

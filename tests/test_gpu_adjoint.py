@@ -1,9 +1,13 @@
 """Adjoint (torch.autograd) checks on the GPU: against the FP64 dense
-reference gradient and against central finite differences (FP64, eps 1e-6).
+reference gradient and against central finite differences (FP64).
 
 Tolerance: the adjoint gradient must agree with central FD to <= 1e-6
-relative in every direction (FD truncation is O(eps^2) = 1e-12 relative; the
-FP64 solves run to 1e-14)."""
+relative in every direction.  Central FD carries a truncation error of order
+eps^2 * ||A^-1 D||^2 (D the value perturbation); for the Zipf matrix below
+(values spanning 1.5e-5..34) that is ~1e-6 at eps=1e-6 and falls 100x per 10x
+smaller eps (checked with dense FP64 solves).  The FD here takes x+ - x- from
+the resolvent identity, which has no cancellation, so eps=1e-7 is used: the
+remaining FD error is ~1e-8 relative, set by the 1e-14 solves."""
 import numpy as np
 import pytest
 
@@ -58,11 +62,13 @@ def test_nonsymmetric_vpgmres_forward_gmres_adjoint(ctx, case):
     if case == "convdiff":
         ptr, col, val = gen.stencil(3, 14, 14, 14, pe=0.8)
     else:
-        ptr, col, val = gen.zipf(3000, 1.5, 300, (51, 52, 53))
+        # cap 32: the full Zipf cap (4096) spreads row scales ~3600x and plain
+        # GMRES needs >10^4 steps to 1e-14 even in FP64 (oracle)
+        ptr, col, val = gen.zipf(3000, 1.5, 32, (51, 52, 53))
     n = len(ptr) - 1
     b = torch.from_numpy(gen.vector(50, n, "uniform")).cuda()
     xt = torch.from_numpy(gen.vector(5, n, "normal")).cuda()
-    fwd = dict(outer_eps=1e-14, outer_maxloop=2000, outer_restart=21, inner_maxloop=11, inner_restart=12,
+    fwd = dict(outer_eps=1e-14, outer_maxloop=3000, outer_restart=21, inner_maxloop=11, inner_restart=12,
                inner_precision=32)
     sysm = AG.SparseSystem(ctx, ptr, col, spd=False, solver="vpgmres", inner="gmres", fp32=True, params=fwd,
                            adjoint_solver="gmres", adjoint_inner="NO",
@@ -78,7 +84,7 @@ def test_nonsymmetric_vpgmres_forward_gmres_adjoint(ctx, case):
     rng = np.random.default_rng(54)
     for _ in range(6):
         dv = torch.from_numpy(rng.standard_normal(len(val))).cuda()
-        fd, naive = AG.central_fd(ctx, ptr, col, vals.detach(), dv, b, xt, 1e-6, "vpgmres", "gmres",
+        fd, naive = AG.central_fd(ctx, ptr, col, vals.detach(), dv, b, xt, 1e-7, "vpgmres", "gmres",
                                   params=fwd)
         ad = float(vals.grad @ dv)
         assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad, naive)
