@@ -63,6 +63,12 @@ typedef enum ks_status {
 #define KS_MAT_SPD      16u /* caller asserts A is symmetric positive definite */
 #define KS_MAT_MERGE    32u /* force the merge-path SpMV for this matrix */
 #define KS_MAT_VECTOR   64u /* force the lanes-per-row vector SpMV */
+#define KS_MAT_WG_TILES 128u /* CSR-stream tiles per workgroup (256 rows / 2048
+                                entries) instead of per wavefront (64 / 512) */
+#define KS_MAT_BORROW   256u /* with KS_PTR_DEVICE: use the caller's ptr/col/val
+                                in place (no copy); they must outlive the
+                                matrix, and values changed in place are made
+                                visible with ks_mat_update_values(m, val) */
 
 /* precision selectors */
 #define KS_F64 64
@@ -139,7 +145,8 @@ ks_status ks_ctx_synchronize(ks_ctx* ctx);
  * Replaces collection::CRSAlloc_Part2 + CudaCopy_Part1 + transposeMatrix
  * (solver_collection.cpp:857-868,932-942,1021-1125) and cuda::CSR2CSC
  * (cudaFunction.cu:896-931): 0-based int32 CSR, N==M, values FP64.  The
- * library copies the arrays to HBM (owned).  KS_MAT_TRANSPOSE builds A^T on
+ * library copies the arrays to HBM (owned), or with KS_MAT_BORROW uses the
+ * caller's device arrays in place.  KS_MAT_TRANSPOSE builds A^T on
  * the device (count, scan, scatter, per-column sort), KS_MAT_FP32_COPY
  * keeps FP32 values for mixed-precision inner solves. */
 ks_status ks_mat_create_csr(ks_ctx* ctx, int64_t n, int64_t nnz,

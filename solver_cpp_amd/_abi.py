@@ -25,6 +25,8 @@ KS_MAT_TRANSPOSE = 8
 KS_MAT_SPD = 16
 KS_MAT_MERGE = 32
 KS_MAT_VECTOR = 64
+KS_MAT_WG_TILES = 128
+KS_MAT_BORROW = 256
 
 KS_F64 = 64
 KS_F32 = 32

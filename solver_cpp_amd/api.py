@@ -172,7 +172,12 @@ class Context:
 class Matrix:
     """CSR matrix in HBM (collection's Cval/Ccol/Cptr + CTval/... for A^T)."""
 
-    def __init__(self, ctx, ptr, col, val, transpose=False, fp32=False, spd=False, merge=False):
+    # SpMV layouts: CSR-stream wavefront tiles (default), CSR-stream workgroup
+    # tiles, lanes-per-row vector kernel, merge-path
+    LAYOUTS = {"stream": 0, "wg": _abi.KS_MAT_WG_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE}
+
+    def __init__(self, ctx, ptr, col, val, transpose=False, fp32=False, spd=False, layout="stream",
+                 borrow=False):
         self.ctx = ctx
         ap, ac, av = _Arg(ptr, np.int32), _Arg(col, np.int32), _Arg(val, np.float64)
         self.n = (ap.obj.numel() if ap.dev else ap.obj.size) - 1
@@ -184,8 +189,13 @@ class Matrix:
             fl |= _abi.KS_MAT_FP32_COPY
         if spd:
             fl |= _abi.KS_MAT_SPD
-        if merge:
-            fl |= _abi.KS_MAT_MERGE
+        if layout not in self.LAYOUTS:
+            raise ValueError(f"layout must be one of {sorted(self.LAYOUTS)}")
+        fl |= self.LAYOUTS[layout]
+        if borrow:
+            # the library reads these device tensors in place: keep them alive
+            fl |= _abi.KS_MAT_BORROW
+            self._borrowed = (ap.obj, ac.obj, av.obj)
         h = C.c_void_p()
         ctx.check(ctx.lib.ks_mat_create_csr(ctx.h, self.n, self.nnz, ap.ptr, ac.ptr, av.ptr, fl, C.byref(h)),
                   "ks_mat_create_csr")

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -41,14 +42,21 @@ template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  bool own = true;  // false: caller's memory (borrow), never freed here
   DBuf() = default;
   explicit DBuf(size_t m) { alloc(m); }
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
-  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), own(o.own) { o.p = nullptr; o.n = 0; o.own = true; }
   DBuf& operator=(DBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) { release(); p = o.p; n = o.n; own = o.own; o.p = nullptr; o.n = 0; o.own = true; }
     return *this;
+  }
+  void borrow(T* q, size_t m) {
+    release();
+    p = q;
+    n = m;
+    own = false;
   }
   ~DBuf() { release(); }
   void alloc(size_t m) {
@@ -64,9 +72,10 @@ struct DBuf {
     n = m;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    own = true;
   }
 };
 
@@ -132,18 +141,26 @@ struct ks_ctx {
   int num_cus = 256;
   ks::Prof prof;
   std::vector<hipEvent_t> evpool;
+  // resident workgroups per CU of each kernel (and dynamic LDS size) launched
+  // on this context's device
+  std::map<std::pair<const void*, size_t>, int> occ;
 };
 
 namespace ks {
 
-// SpMV plan of one CSR matrix: CSR-stream tiles (kind 0) or L lanes per
-// row (kind 1).
+// SpMV plan of one CSR matrix: CSR-stream tiles (kind 0), L lanes per row
+// (kind 1) or merge-path segments (kind 2).  For kinds 0 and 1 the tiles are
+// wavefront tiles (k_spmv_wave) unless wg_tiles (workgroup tiles,
+// k_spmv_stream); the two-input SpMV of kind 1 uses the tiles too.  For kind
+// 2, tiles / tnz hold the segment start points (row, entry) of the merge path.
 struct Plan {
   int kind = 0;
+  bool wg_tiles = false;
   int L = 8;
   int64_t ntiles = 0;
   DBuf<int32_t> tiles;  // tile row starts, ntiles + 1
   DBuf<int32_t> tnz;    // tile entry starts (ptr[tiles[t]]), ntiles + 1
+  DBuf<double> carry;   // merge path: 2 * ntiles carries (FP64 or FP32)
 };
 
 template <class T>
@@ -153,10 +170,12 @@ struct MatView {
   const int32_t* col = nullptr;
   const T* val = nullptr;
   int kind = 0;
+  bool wg_tiles = false;
   int L = 8;
   int64_t ntiles = 0;
   const int32_t* tiles = nullptr;
   const int32_t* tnz = nullptr;
+  T* carry = nullptr;
 };
 
 }  // namespace ks
@@ -186,7 +205,8 @@ inline ks::MatView<double> ks_mat::view<double>(bool trans) const {
   const ks::Plan& pl = trans ? tplan : plan;
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val.p; }
-  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
+  v.kind = pl.kind; v.wg_tiles = pl.wg_tiles; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
+  v.carry = reinterpret_cast<decltype(v.carry)>(pl.carry.p);
   return v;
 }
 template <>
@@ -196,7 +216,8 @@ inline ks::MatView<float> ks_mat::view<float>(bool trans) const {
   const ks::Plan& pl = trans ? tplan : plan;
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval32.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val32.p; }
-  v.kind = pl.kind; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
+  v.kind = pl.kind; v.wg_tiles = pl.wg_tiles; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
+  v.carry = reinterpret_cast<decltype(v.carry)>(pl.carry.p);
   return v;
 }
 
@@ -222,6 +243,32 @@ inline int grid_rows(int64_t nrb) {
   return (int)g;
 }
 
+// Workgroups of `kernel` that are resident on the whole device at once.  The
+// grid-stride kernels are launched with at most this many workgroups: a grid
+// larger than one resident wave leaves a second, partly filled wave of
+// workgroups (a tail) that runs after the rest.
+template <class K>
+int resident(ks_ctx* c, K kernel, size_t dyn_lds = 0) {
+  const auto key = std::make_pair(reinterpret_cast<const void*>(kernel), dyn_lds);
+  auto it = c->occ.find(key);
+  if (it != c->occ.end()) return it->second;
+  int per_cu = 0;
+  KS_HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), kBlock,
+                                                         dyn_lds));
+  const int r = std::max(1, per_cu) * std::max(1, c->num_cus);
+  c->occ.emplace(key, r);
+  return r;
+}
+// grid_map / grid_rows, capped at one resident wave of `kernel`
+template <class K>
+int grid_map(ks_ctx* c, K kernel, int64_t n, size_t dyn_lds = 0) {
+  return grid_rows(std::min<int64_t>(grid_map(n), resident(c, kernel, dyn_lds)));
+}
+template <class K>
+int grid_rows(ks_ctx* c, K kernel, int64_t nrb, size_t dyn_lds = 0) {
+  return grid_rows(std::min<int64_t>(nrb, resident(c, kernel, dyn_lds)));
+}
+
 // Finisher of a reduction whose producer ran on G workgroups.
 template <int NV, class F>
 void launch_finish(ks_ctx* c, int G, const F& f, Gate g) {
@@ -231,7 +278,7 @@ void launch_finish(ks_ctx* c, int G, const F& f, Gate g) {
 
 template <int NV, class Op>
 void launch_map(ks_ctx* c, int64_t n, const Op& op, Gate g, int cls, double bytes) {
-  const int G = grid_map(n);
+  const int G = grid_map(c, k_map<NV, Op>, n);
   c->prof.begin(c->stream, cls);
   k_map<NV, Op><<<G, kBlock, 0, c->stream>>>(n, op, g, red(c), stat(c, cls, bytes));
   c->prof.end(c->stream);
@@ -255,7 +302,7 @@ template <class T, int L, int NV, class Epi>
 int spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, int cls, double bytes) {
   constexpr int RPB = kBlock / L;
   const int64_t nrb = (A.n + RPB - 1) / RPB;
-  const int G = grid_rows(nrb);
+  const int G = grid_rows(c, k_spmv<T, L, NV, Epi>, nrb);
   c->prof.begin(c->stream, cls);
   k_spmv<T, L, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ptr, A.col, A.val, x, epi, g, red(c),
                                                      stat(c, cls, bytes));
@@ -267,10 +314,35 @@ int spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, i
 template <class T, int NX, int NV, class Epi>
 int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
                 double bytes) {
-  const int G = grid_rows(A.ntiles);
+  int G;
   c->prof.begin(c->stream, cls);
-  k_spmv_stream<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val,
+  if (A.wg_tiles) {
+    G = grid_rows(c, k_spmv_stream<T, NX, NV, Epi>, A.ntiles);
+    k_spmv_stream<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val,
+                                                               x0, x1, epi, g, red(c), stat(c, cls, bytes));
+  } else {
+    G = grid_rows(c, k_spmv_wave<T, NX, NV, Epi>, (A.ntiles + kBlock / 64 - 1) / (kBlock / 64));
+    k_spmv_wave<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val,
                                                              x0, x1, epi, g, red(c), stat(c, cls, bytes));
+  }
+  c->prof.end(c->stream);
+  KS_HIPCK(hipGetLastError());
+  return G;
+}
+
+// Merge-path SpMV: carries of the split rows, then the segments.
+template <class T, int NX, int NV, class Epi>
+int spmv_merge(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
+               double bytes) {
+  const int64_t nslot = (A.ntiles + kBlock / 64 - 1) / (kBlock / 64);
+  c->prof.begin(c->stream, cls);
+  const int Gc = grid_rows(c, k_merge_carry<T, NX>, nslot);
+  k_merge_carry<T, NX><<<Gc, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
+                                                    A.carry, g);
+  KS_HIPCK(hipGetLastError());
+  const int G = grid_rows(c, k_spmv_merge<T, NX, NV, Epi>, nslot);
+  k_spmv_merge<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0,
+                                                            x1, A.carry, epi, g, red(c), stat(c, cls, bytes));
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
   return G;
@@ -284,6 +356,8 @@ void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gat
   int G;
   if (A.kind == 0) {
     G = spmv_stream<T, 1, NV>(c, A, x, x, epi, g, cls, bytes);
+  } else if (A.kind == 2) {
+    G = spmv_merge<T, 1, NV>(c, A, x, x, epi, g, cls, bytes);
   } else {
     switch (A.L) {
       case 2: G = spmv_L<T, 2, NV>(c, A, x, epi, g, cls, bytes); break;
@@ -300,39 +374,72 @@ void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gat
 template <class T>
 struct EpiStore2;
 
-// y1 = A x1 and y2 = A x2 in one pass over the matrix (always CSR-stream).
+// y1 = A x1 and y2 = A x2 in one pass over the matrix (always CSR-stream,
+// with the plan's tiles).
 template <class T>
 void launch_spmv2(ks_ctx* c, const MatView<T>& A, const T* x1, const T* x2, T* y1, T* y2, Gate g) {
   const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * A.n;
-  spmv_stream<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);
+  if (A.kind == 2) spmv_merge<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);
+  else spmv_stream<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);
 }
 
-// CSR-stream tiles from host row pointers.
-inline std::vector<int32_t> build_tiles(const int32_t* hp, int64_t n) {
+// CSR-stream tiles from host row pointers: greedy runs of at most max_rows
+// rows and max_nnz entries; a longer row is a tile of its own.
+inline std::vector<int32_t> build_tiles(const int32_t* hp, int64_t n, int64_t max_rows, int64_t max_nnz) {
   std::vector<int32_t> t;
-  t.reserve(n / 64 + 2);
+  t.reserve(n / 32 + 2);
   t.push_back(0);
   int64_t r = 0;
   while (r < n) {
     const int64_t start = r, j0 = hp[r];
-    if (hp[r + 1] - j0 > kTileNnz) {
+    if (hp[r + 1] - j0 > max_nnz) {
       ++r;
     } else {
-      while (r < n && r - start < kTileRows && hp[r + 1] - j0 <= kTileNnz) ++r;
+      while (r < n && r - start < max_rows && hp[r + 1] - j0 <= max_nnz) ++r;
     }
     t.push_back((int32_t)r);
   }
   return t;
 }
 
-inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t nnz, bool vector) {
+// Merge-path segment starts: the path point (row i, entry j) on each
+// diagonal d = s * items (Merrill & Garland's MergePathSearch with
+// A[i] = ptr[i + 1], B[j] = j: a row end is taken before entry j when
+// ptr[i + 1] <= j).
+inline void merge_segments(const int32_t* hp, int64_t n, int64_t nnz, int64_t items, std::vector<int32_t>& segr,
+                           std::vector<int32_t>& segj) {
+  const int64_t total = n + nnz, nseg = total > 0 ? (total + items - 1) / items : 1;
+  segr.resize(nseg + 1);
+  segj.resize(nseg + 1);
+  for (int64_t s = 0; s <= nseg; ++s) {
+    const int64_t d = std::min(s * items, total);
+    int64_t lo = std::max<int64_t>(d - nnz, 0), hi = std::min<int64_t>(d, n);
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (hp[mid + 1] <= d - mid - 1) lo = mid + 1;
+      else hi = mid;
+    }
+    segr[s] = (int32_t)lo;
+    segj[s] = (int32_t)(d - lo);
+  }
+}
+
+inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t nnz, bool vector, bool wg_tiles,
+                      bool merge = false) {
   pl.L = 2;
   const double mean = n > 0 ? (double)nnz / (double)n : 1.0;
   while (pl.L < 64 && pl.L < mean) pl.L *= 2;
-  pl.kind = vector ? 1 : 0;
-  std::vector<int32_t> t = build_tiles(hp, n);
-  std::vector<int32_t> z(t.size());
-  for (size_t i = 0; i < t.size(); ++i) z[i] = hp[t[i]];
+  pl.kind = merge ? 2 : vector ? 1 : 0;
+  pl.wg_tiles = wg_tiles && !merge;
+  std::vector<int32_t> t, z;
+  if (merge) {
+    merge_segments(hp, n, nnz, kMergeItems, t, z);
+    pl.carry.alloc(2 * (t.size() - 1));
+  } else {
+    t = wg_tiles ? build_tiles(hp, n, kTileRows, kTileNnz) : build_tiles(hp, n, kWaveRows, kWaveNnz);
+    z.resize(t.size());
+    for (size_t i = 0; i < t.size(); ++i) z[i] = hp[t[i]];
+  }
   pl.ntiles = (int64_t)t.size() - 1;
   pl.tiles.alloc(t.size());
   pl.tnz.alloc(t.size());

@@ -155,7 +155,8 @@ void build_transpose(ks_mat* m) {
     k_gather<double, double><<<grid_map(nnz), kBlock, 0, s>>>(nnz, m->tperm.p, m->val.p, m->tval.p);
     KS_HIPCK(hipGetLastError());
   }
-  make_plan(c, m->tplan, htp.data(), n, nnz, (m->flags & KS_MAT_VECTOR) != 0);
+  make_plan(c, m->tplan, htp.data(), n, nnz, (m->flags & KS_MAT_VECTOR) != 0, (m->flags & KS_MAT_WG_TILES) != 0,
+            (m->flags & KS_MAT_MERGE) != 0);
   m->has_t = true;
 }
 
@@ -479,14 +480,22 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   KS_CHECK(nnz >= 0 && nnz < (int64_t)INT32_MAX, KS_ERR_BAD_INPUT, "nnz must be in [0, 2^31)");
   KS_CHECK(ptr && (nnz == 0 || (col && val)), KS_ERR_BAD_INPUT, "null CSR array");
   m->ctx = c; m->n = n; m->nnz = nnz; m->flags = flags;
-  m->ptr.alloc(n + 1);
-  m->col.alloc(std::max<int64_t>(nnz, 1));
-  m->val.alloc(std::max<int64_t>(nnz, 1));
-  const hipMemcpyKind k = (flags & KS_PTR_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  KS_HIPCK(hipMemcpyAsync(m->ptr.p, ptr, sizeof(int32_t) * (n + 1), k, c->stream));
-  if (nnz > 0) {
-    KS_HIPCK(hipMemcpyAsync(m->col.p, col, sizeof(int32_t) * nnz, k, c->stream));
-    KS_HIPCK(hipMemcpyAsync(m->val.p, val, sizeof(double) * nnz, k, c->stream));
+  if (flags & KS_MAT_BORROW) {
+    KS_CHECK(flags & KS_PTR_DEVICE, KS_ERR_BAD_INPUT, "KS_MAT_BORROW needs device arrays (KS_PTR_DEVICE)");
+    KS_CHECK(nnz > 0, KS_ERR_BAD_INPUT, "KS_MAT_BORROW needs nnz > 0");
+    m->ptr.borrow(const_cast<int32_t*>(ptr), n + 1);
+    m->col.borrow(const_cast<int32_t*>(col), nnz);
+    m->val.borrow(const_cast<double*>(val), nnz);
+  } else {
+    m->ptr.alloc(n + 1);
+    m->col.alloc(std::max<int64_t>(nnz, 1));
+    m->val.alloc(std::max<int64_t>(nnz, 1));
+    const hipMemcpyKind k = (flags & KS_PTR_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    KS_HIPCK(hipMemcpyAsync(m->ptr.p, ptr, sizeof(int32_t) * (n + 1), k, c->stream));
+    if (nnz > 0) {
+      KS_HIPCK(hipMemcpyAsync(m->col.p, col, sizeof(int32_t) * nnz, k, c->stream));
+      KS_HIPCK(hipMemcpyAsync(m->val.p, val, sizeof(double) * nnz, k, c->stream));
+    }
   }
   // validate on the host (row pointers, column range)
   std::vector<int32_t> hp(n + 1), hc;
@@ -505,7 +514,8 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   for (int64_t j = 0; j < nnz; ++j)
     KS_CHECK(hc[j] >= 0 && hc[j] < n, KS_ERR_BAD_INPUT, "column index out of range at entry " + std::to_string(j));
   m->max_row = mx;
-  make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0);
+  make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0, (flags & KS_MAT_WG_TILES) != 0,
+              (flags & KS_MAT_MERGE) != 0);
   if (flags & KS_MAT_TRANSPOSE) build_transpose(m.get());
   if (flags & KS_MAT_FP32_COPY) ensure_fp32(m.get());
   KS_HIPCK(hipStreamSynchronize(c->stream));
@@ -520,8 +530,13 @@ ks_status ks_mat_update_values(ks_mat* m, const double* val, uint32_t flags) {
   KS_API_TRY
   set_device(c);
   if (m->nnz == 0) return KS_OK;
-  const hipMemcpyKind k = (flags & KS_PTR_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  KS_HIPCK(hipMemcpyAsync(m->val.p, val, sizeof(double) * m->nnz, k, c->stream));
+  KS_CHECK(val, KS_ERR_BAD_INPUT, "null values");
+  // a borrowed matrix updated in place passes its own value array: only the
+  // derived copies (A^T values, FP32 values) are refreshed
+  if (val != m->val.p) {
+    const hipMemcpyKind k = (flags & KS_PTR_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    KS_HIPCK(hipMemcpyAsync(m->val.p, val, sizeof(double) * m->nnz, k, c->stream));
+  }
   if (m->has_t) {
     k_gather<double, double><<<grid_map(m->nnz), kBlock, 0, c->stream>>>(m->nnz, m->tperm.p, m->val.p, m->tval.p);
     KS_HIPCK(hipGetLastError());

@@ -189,9 +189,12 @@ __global__ void k_scalar(Op op, Gate g, StatArgs st) {
 
 // -------------------------------------------------------------- SpMV ----
 // Row-block r of `nrb` for workgroup slot `it`: XCD-contiguous ranges.
+// With nsub > 1 each workgroup is split into nsub slots (its wavefronts),
+// slot `sub` of workgroup b taking every (slots per XCD)-th block of the
+// XCD's range.
 struct RowBlocks {
   int64_t nrb;
-  __device__ void range(int64_t& first, int64_t& end, int64_t& step) const {
+  __device__ void range(int64_t& first, int64_t& end, int64_t& step, int sub = 0, int nsub = 1) const {
     const int G = gridDim.x;
     if (G % kXcd == 0 && G >= 8 * kXcd) {
       const int xcd = blockIdx.x % kXcd, bi = blockIdx.x / kXcd, gx = G / kXcd;
@@ -199,13 +202,13 @@ struct RowBlocks {
       const int64_t lo = (int64_t)xcd * per;
       int64_t hi = lo + per;
       if (hi > nrb) hi = nrb;
-      first = lo + bi;
+      first = lo + (int64_t)bi * nsub + sub;
       end = hi;
-      step = gx;
+      step = (int64_t)gx * nsub;
     } else {
-      first = blockIdx.x;
+      first = (int64_t)blockIdx.x * nsub + sub;
       end = nrb;
-      step = G;
+      step = (int64_t)G * nsub;
     }
   }
 };
@@ -288,8 +291,15 @@ struct TileData {
   }
 };
 
+// Register budget of the CSR-stream kernels as waves/SIMD: FP32 64 VGPRs
+// (8 waves) for a plain SpMV and 72 (7 waves) with a fused reduction, FP64
+// 96 (5 waves) and 128 (4 waves) for the two-input SpMV - the largest
+// occupancy each reaches without spilling.
+template <class T, int NX, int NV>
+constexpr int kStreamWaves = sizeof(T) == 4 ? (NV == 0 ? 8 : 7) : (NX == 2 ? 4 : 5);
+
 template <class T, int NX, int NV, class Epi>
-__global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamWaves<T, NX, NV>))) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
                                                         const int32_t* __restrict__ tnz,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ col,
@@ -397,6 +407,137 @@ __global__ __launch_bounds__(kBlock) void k_spmv_stream(int64_t n, int64_t ntile
   if constexpr (NV > 0) block_store<NV>(acc, ra);
 }
 
+
+// Wavefront CSR-stream: the scheme of k_spmv_stream with every wavefront
+// working alone on tiles of at most kWaveRows rows and kWaveNnz entries
+// (8 per lane), so no workgroup barrier sits in the tile loop and one slow
+// gather stalls only its own wavefront.  A wavefront's LDS instructions
+// execute in order, so wavefront-scope fences (which keep the compiler from
+// moving LDS accesses across them) order its product writes before its row
+// sums and its sums before the next tile's writes.
+constexpr int kWaveNnz = 512;
+constexpr int kWaveRows = 64;
+constexpr int kWaveUnroll = kWaveNnz / 64;
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+struct WaveTile {
+  T v[kWaveUnroll];
+  int c[kWaveUnroll];
+  int32_t pr;  // row pointer of row (r0 + lane)
+  __device__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                       const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1, int lane) {
+    const int cnt = j1 - j0;
+    if (cnt > kWaveNnz) return;  // long row: loaded in place
+#pragma unroll
+    for (int q = 0; q < kWaveUnroll; ++q) {
+      const int k = lane + q * 64;
+      if (k < cnt) {
+        v[q] = ld_stream(val + j0 + k);
+        c[q] = ld_stream(col + j0 + k);
+      }
+    }
+    pr = lane < r1 - r0 ? ptr[r0 + lane] : 0;
+  }
+};
+
+template <class T, int NX, int NV, class Epi>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamWaves<T, NX, NV>))) void k_spmv_wave(
+    int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles, const int32_t* __restrict__ tnz,
+    const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* __restrict__ x0, const T* __restrict__ x1, Epi epi, Gate g, RedArgs ra, StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  epi.prep();
+  constexpr int kW = kBlock / 64;
+  __shared__ T prod[NX][kW][kWaveNnz];
+  __shared__ int32_t sptr_all[kW][kWaveRows + 1];
+  double acc[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < (NV > 0 ? NV : 1); ++i) acc[i] = 0.0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T* p0 = prod[0][w];
+  T* p1 = prod[NX - 1][w];
+  int32_t* sptr = sptr_all[w];
+  RowBlocks rbs{ntiles};
+  int64_t t, end, step;
+  rbs.range(t, end, step, w, kW);
+  int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
+  WaveTile<T> d;
+  if (t < end) {
+    r0 = tiles[t]; r1 = tiles[t + 1]; j0 = tnz[t]; j1 = tnz[t + 1];
+    d.load(ptr, col, val, r0, r1, j0, j1, lane);
+  }
+  int32_t nr0 = 0, nr1 = 0, nj0 = 0, nj1 = 0;
+  if (t + step < end) {
+    nr0 = tiles[t + step]; nr1 = tiles[t + step + 1]; nj0 = tnz[t + step]; nj1 = tnz[t + step + 1];
+  }
+  for (; t < end; t += step) {
+    const int nr = r1 - r0, cnt = j1 - j0;
+    const int64_t tn = t + step, tnn = t + 2 * step;
+    const bool more = tn < end, more2 = tnn < end;
+    int32_t mr0 = 0, mr1 = 0, mj0 = 0, mj1 = 0;
+    if (cnt > kWaveNnz) {
+      // one long row: lane-strided partial sums, then a fixed shuffle tree
+      T s0 = 0, s1 = 0;
+      for (int k = lane; k < cnt; k += 64) {
+        const T a = ld_stream(val + j0 + k);
+        const int c = ld_stream(col + j0 + k);
+        s0 += a * x0[c];
+        if (NX == 2) s1 += a * x1[c];
+      }
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, lane);
+      if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        if (NX == 2) s1 += __shfl_xor(s1, off, 64);
+      }
+      if (lane == 0) epi.row(r0, s0, s1, acc);
+    } else {
+      if (lane < nr) sptr[lane] = d.pr - j0;
+      if (lane == 0) sptr[nr] = cnt;
+#pragma unroll
+      for (int q = 0; q < kWaveUnroll; ++q) {
+        const int k = lane + q * 64;
+        if (k < cnt) {
+          p0[k] = d.v[q] * x0[d.c[q]];
+          if (NX == 2) p1[k] = d.v[q] * x1[d.c[q]];
+        }
+      }
+      wave_sync();
+      // in flight during the sums: the next tile's data, then the bounds after it
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, lane);
+      if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
+      // more than one lane per row only for rows averaging > 8 entries
+      int tpr = 1;
+      while (tpr < 64 && tpr * 2 * nr <= 64 && tpr * 8 * nr < cnt) tpr *= 2;
+      const int grp = lane / tpr, sub = lane % tpr;
+      T a0 = 0, a1 = 0;
+      if (grp < nr) {
+        const int s = sptr[grp], e = sptr[grp + 1];
+        for (int k = s + sub; k < e; k += tpr) {
+          a0 += p0[k];
+          if (NX == 2) a1 += p1[k];
+        }
+      }
+      for (int off = tpr >> 1; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        if (NX == 2) a1 += __shfl_xor(a1, off, 64);
+      }
+      if (grp < nr && sub == 0) epi.row(r0 + grp, a0, a1, acc);
+      wave_sync();
+    }
+    r0 = nr0; r1 = nr1; j0 = nj0; j1 = nj1;
+    nr0 = mr0; nr1 = mr1; nj0 = mj0; nj1 = mj1;
+  }
+  if constexpr (NV > 0) block_store<NV>(acc, ra);
+}
 // Two products with one pass over the matrix: y1 = A x1, y2 = A x2
 // (matrix-powers basis of the k-skip solvers).
 template <class T, int L>
@@ -473,9 +614,146 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(int64_t n, const int32_t* __re
   }
 }
 
+// ---------------------------------------------------- merge-path SpMV --
+// The merge path of the row ends (ptr[1..n]) and the entries (0..nnz) is cut
+// into segments of kMergeItems items (Merrill & Garland, "Merge-based
+// parallel sparse matrix-vector multiplication", SC16); segment s starts at
+// path point (segr[s], segj[s]) (computed on the host).  A row belongs to the
+// segment in which it ends; the entries of a row that lie in earlier segments
+// are summed by those segments as carries: carry[s] is the partial sum of the
+// row still open at the end of segment s (k_merge_carry).  k_spmv_merge then
+// sums each segment's own rows from LDS products, a row continued from
+// earlier segments starting from its carries in segment order, so every row
+// total is formed in a fixed order and passes once through the epilogue.
+constexpr int kMergeItems = 512;
+constexpr int kMergeUnroll = kMergeItems / 64;
+
+template <class T, int NX>
+__global__ __launch_bounds__(kBlock) void k_merge_carry(int64_t n, int64_t nseg, const int32_t* __restrict__ segr,
+                                                        const int32_t* __restrict__ segj,
+                                                        const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                        const T* __restrict__ x0, const T* __restrict__ x1,
+                                                        T* __restrict__ carry, Gate g) {
+  if (g.off()) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  RowBlocks rbs{nseg};
+  int64_t s, end, step;
+  rbs.range(s, end, step, w, kBlock / 64);
+  for (; s < end; s += step) {
+    const int32_t r = segr[s + 1], js = segj[s], je = segj[s + 1];
+    T s0 = 0, s1 = 0;
+    if (r < n) {
+      const int32_t a = ptr[r] > js ? ptr[r] : js;
+      for (int32_t k = a + lane; k < je; k += 64) {
+        const T v = ld_stream(val + k);
+        const int c = ld_stream(col + k);
+        s0 += v * x0[c];
+        if (NX == 2) s1 += v * x1[c];
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      if (NX == 2) s1 += __shfl_xor(s1, off, 64);
+    }
+    if (lane == 0) {
+      carry[s] = s0;
+      if (NX == 2) carry[nseg + s] = s1;
+    }
+  }
+}
+
+template <class T, int NX, int NV, class Epi>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamWaves<T, NX, NV>))) void k_spmv_merge(
+    int64_t n, int64_t nseg, const int32_t* __restrict__ segr, const int32_t* __restrict__ segj,
+    const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* __restrict__ x0, const T* __restrict__ x1, const T* __restrict__ carry, Epi epi, Gate g, RedArgs ra,
+    StatArgs st) {
+  if (g.off()) return;
+  st.count();
+  epi.prep();
+  constexpr int kW = kBlock / 64;
+  __shared__ T prod[NX][kW][kMergeItems];
+  double acc[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < (NV > 0 ? NV : 1); ++i) acc[i] = 0.0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T* p0 = prod[0][w];
+  T* p1 = prod[NX - 1][w];
+  RowBlocks rbs{nseg};
+  int64_t s, end, step;
+  rbs.range(s, end, step, w, kW);
+  for (; s < end; s += step) {
+    const int32_t r0 = segr[s], r1 = segr[s + 1], js = segj[s], je = segj[s + 1];
+    // own entries: up to the start of the open row r1 (summed as carry[s])
+    int32_t e = je;
+    if (r1 < n) {
+      const int32_t pr1 = ptr[r1];
+      e = pr1 > js ? pr1 : js;
+    }
+    const int cnt = e - js;
+#pragma unroll
+    for (int q = 0; q < kMergeUnroll; ++q) {
+      const int k = lane + q * 64;
+      if (k < cnt) {
+        const T v = ld_stream(val + js + k);
+        const int c = ld_stream(col + js + k);
+        p0[k] = v * x0[c];
+        if (NX == 2) p1[k] = v * x1[c];
+      }
+    }
+    wave_sync();
+    const int nr = r1 - r0;
+    int tpr = 1;
+    while (tpr < 64 && tpr * 2 * nr <= 64 && tpr * 8 * nr < cnt) tpr *= 2;
+    const int sub = lane % tpr;
+    for (int base = 0; base < nr; base += 64 / tpr) {
+      const int grp = base + lane / tpr;
+      const bool live = grp < nr;
+      const int32_t r = r0 + grp;
+      T a0 = 0, a1 = 0, c0 = 0, c1 = 0;
+      if (live) {
+        const int32_t pa = ptr[r], pb = ptr[r + 1];
+        if (grp == 0 && pa < js && sub == 0) {
+          // carries of the earlier segments that ended inside row r0, in order
+          int64_t f = s;
+          while (f > 0 && segr[f] == r0) --f;
+          for (int64_t q = f; q < s; ++q) {
+            c0 += carry[q];
+            if (NX == 2) c1 += carry[nseg + q];
+          }
+        }
+        const int sa = (pa > js ? pa : js) - js, sb = pb - js;
+        for (int k = sa + sub; k < sb; k += tpr) {
+          a0 += p0[k];
+          if (NX == 2) a1 += p1[k];
+        }
+      }
+      for (int off = tpr >> 1; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        if (NX == 2) a1 += __shfl_xor(a1, off, 64);
+      }
+      if (live && sub == 0) epi.row(r, c0 + a0, c1 + a1, acc);
+    }
+    wave_sync();
+  }
+  if constexpr (NV > 0) block_store<NV>(acc, ra);
+}
+
 // ---------------------------------------------------- Gram-Schmidt ------
+// 16-byte vectors of T (float4 / double2) for the column-streaming kernels:
+// with VEC, thread i reads elements [i*N, i*N + N) of every column with one
+// 16-byte load (the columns and w must be 16-byte aligned and ldv a multiple
+// of N); the n % N tail elements are taken by the first threads.
+template <class T>
+struct Vec16 {
+  static constexpr int N = 16 / sizeof(T);
+  typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+
 // h[i] = (w, V_{i0+i}) for i < nv (nv <= G): one pass over w and nv columns.
-template <class T, int G>
+template <class T, int G, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_proj(int64_t n, const T* __restrict__ V, int64_t ldv, int nv,
                                                  const T* __restrict__ w, Gate g, RedArgs ra, StatArgs st) {
   if (g.off()) return;
@@ -484,18 +762,44 @@ __global__ __launch_bounds__(kBlock) void k_proj(int64_t n, const T* __restrict_
 #pragma unroll
   for (int v = 0; v < G; ++v) acc[v] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  for (; i + stride < n; i += 2 * stride) {
-    const double wa = (double)w[i], wb = (double)w[i + stride];
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if constexpr (VEC) {
+    using VT = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::N;
+    const int64_t nvec = n / N;
+    const VT* wv = reinterpret_cast<const VT*>(w);
+    for (int64_t i = gid; i < nvec; i += stride) {
+      const VT a = wv[i];
 #pragma unroll
-    for (int v = 0; v < G; ++v)
-      if (v < nv) acc[v] += wa * (double)V[(int64_t)v * ldv + i] + wb * (double)V[(int64_t)v * ldv + i + stride];
-  }
-  for (; i < n; i += stride) {
-    const double wi = (double)w[i];
+      for (int v = 0; v < G; ++v)
+        if (v < nv) {
+          const VT b = reinterpret_cast<const VT*>(V + (int64_t)v * ldv)[i];
+          double t = 0.0;
 #pragma unroll
-    for (int v = 0; v < G; ++v)
-      if (v < nv) acc[v] += wi * (double)V[(int64_t)v * ldv + i];
+          for (int e = 0; e < N; ++e) t += (double)a[e] * (double)b[e];
+          acc[v] += t;
+        }
+    }
+    for (int64_t i = nvec * N + gid; i < n; i += stride) {
+      const double wi = (double)w[i];
+#pragma unroll
+      for (int v = 0; v < G; ++v)
+        if (v < nv) acc[v] += wi * (double)V[(int64_t)v * ldv + i];
+    }
+  } else {
+    int64_t i = gid;
+    for (; i + stride < n; i += 2 * stride) {
+      const double wa = (double)w[i], wb = (double)w[i + stride];
+#pragma unroll
+      for (int v = 0; v < G; ++v)
+        if (v < nv) acc[v] += wa * (double)V[(int64_t)v * ldv + i] + wb * (double)V[(int64_t)v * ldv + i + stride];
+    }
+    for (; i < n; i += stride) {
+      const double wi = (double)w[i];
+#pragma unroll
+      for (int v = 0; v < G; ++v)
+        if (v < nv) acc[v] += wi * (double)V[(int64_t)v * ldv + i];
+    }
   }
   block_store<G>(acc, ra);
 }
@@ -509,7 +813,7 @@ struct FinProj {
 };
 
 // w -= sum_{i<nv} h_i V_i (classical GS update), then ||w||^2 -> fin.
-template <class T>
+template <class T, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __restrict__ V, int64_t ldv,
                                                        int nv, T* __restrict__ w, const double* hcol,
                                                        int ldh, Gate g, RedArgs ra, StatArgs st) {
@@ -520,7 +824,36 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
   __syncthreads();
   double acc[1] = {0.0};
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t i0 = gid;
+  if constexpr (VEC) {
+    using VT = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::N;
+    const int64_t nvec = n / N;
+    VT* wv = reinterpret_cast<VT*>(w);
+    for (int64_t i = gid; i < nvec; i += stride) {
+      const VT a = wv[i];
+      double t[N];
+#pragma unroll
+      for (int e = 0; e < N; ++e) t[e] = (double)a[e];
+#pragma unroll 4
+      for (int v = 0; v < nv; ++v) {
+        const VT b = reinterpret_cast<const VT*>(V + (int64_t)v * ldv)[i];
+        const double h = sh[v];
+#pragma unroll
+        for (int e = 0; e < N; ++e) t[e] -= h * (double)b[e];
+      }
+      VT o;
+#pragma unroll
+      for (int e = 0; e < N; ++e) {
+        o[e] = (T)t[e];
+        acc[0] += (double)o[e] * (double)o[e];
+      }
+      wv[i] = o;
+    }
+    i0 = nvec * N + gid;
+  }
+  for (int64_t i = i0; i < n; i += stride) {
     double t = (double)w[i];
 #pragma unroll 8
     for (int v = 0; v < nv; ++v) t -= sh[v] * (double)V[(int64_t)v * ldv + i];
@@ -616,21 +949,45 @@ __global__ void k_mark_done(Ctl* c, Gate g) {
 // Givens rotations for column k (gmres.cpp:384-401), then the stop test of
 // step k+1 when it lies in this cycle (gmres.cpp:278-324).  Runs in the
 // finisher of the Gram-Schmidt norm, right after h_{k+1,k} is known.
+// The column is rotated in chunks of 8: the chunk's H entries, cosines and
+// sines are loaded together (H, cs, sn, e do not overlap), so a step costs
+// one memory round trip per 8 rotations instead of one per rotation.
 __device__ inline void givens_step(Ctl* c, int k) {
-  double* H = c->H;
+  double* __restrict__ H = c->H;
+  double* __restrict__ cs = c->cs;
+  double* __restrict__ sn = c->sn;
+  double* __restrict__ e = c->e;
   const int ld = c->ld;
-  for (int i = 0; i <= k - 1; ++i) {
-    const double t1 = H[i * ld + k], t2 = H[(i + 1) * ld + k];
-    H[i * ld + k] = c->cs[i] * t1 - c->sn[i] * t2;
-    H[(i + 1) * ld + k] = c->sn[i] * t1 + c->cs[i] * t2;
+  const double hk1 = H[(k + 1) * ld + k];
+  const double ek = e[k];
+  double t1 = H[k];  // H[0][k], then the partly rotated H[i][k]
+  for (int i0 = 0; i0 < k; i0 += 8) {
+    double t2[8], cv[8], sv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (i0 + j < k) {
+        t2[j] = H[(i0 + j + 1) * ld + k];
+        cv[j] = cs[i0 + j];
+        sv[j] = sn[i0 + j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (i0 + j < k) {
+        H[(i0 + j) * ld + k] = cv[j] * t1 - sv[j] * t2[j];
+        t1 = sv[j] * t1 + cv[j] * t2[j];
+      }
+    }
   }
-  const double hk = H[k * ld + k], hk1 = H[(k + 1) * ld + k];
+  // t1 is now the rotated H[k][k]
+  const double hk = t1;
   const double alpha = sqrt(hk * hk + hk1 * hk1);
-  c->cs[k] = hk / alpha;
-  c->sn[k] = -hk1 / alpha;
-  c->e[k + 1] = c->sn[k] * c->e[k];
-  c->e[k] = c->cs[k] * c->e[k];
-  H[k * ld + k] = c->cs[k] * H[k * ld + k] - c->sn[k] * H[(k + 1) * ld + k];
+  const double ck = hk / alpha, sk = -hk1 / alpha;
+  cs[k] = ck;
+  sn[k] = sk;
+  e[k + 1] = sk * ek;
+  e[k] = ck * ek;
+  H[k * ld + k] = ck * hk - sk * hk1;
   H[(k + 1) * ld + k] = 0.0;
   c->count++;
   c->steps++;

@@ -626,12 +626,24 @@ struct BicgLevel : Level {
 
 // --------------------------------------------------- projection helper --
 // h[i] = (w, V_i), i <= k, in launches of up to 32 columns.
+// 16-byte loads need 16-byte aligned columns (base and ldv).
+template <class T>
+bool vec_ok(const T* V, int64_t ldv, const T* w) {
+  return ((uintptr_t)V % 16 == 0) && ((uintptr_t)w % 16 == 0) && (ldv % Vec16<T>::N == 0);
+}
+
 template <class T, int GV>
 void proj_launch(ks_ctx* c, int64_t n, const T* Vi, int64_t ldv, int m, const T* w, double* hi, int ldh, Gate g) {
-  const int G = grid_map(n);
   const double bytes = (double)sizeof(T) * n * (m + 1);
+  int G;
   c->prof.begin(c->stream, KS_K_CGS);
-  k_proj<T, GV><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
+  if (vec_ok(Vi, ldv, w)) {
+    G = grid_map(c, k_proj<T, GV, true>, (n + Vec16<T>::N - 1) / Vec16<T>::N);
+    k_proj<T, GV, true><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
+  } else {
+    G = grid_map(c, k_proj<T, GV, false>, n);
+    k_proj<T, GV, false><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
+  }
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
   launch_finish<GV>(c, G, FinProj{hi, ldh, m}, g);
@@ -661,10 +673,18 @@ template <class T, class Fin>
 void launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, T* w, const double* hcol, int ldh,
                        Fin f, Gate g) {
   const double bytes = (double)sizeof(T) * n * (nv + 2);
-  const int G = grid_map(n);
+  const size_t lds = sizeof(double) * std::max(nv, 1);
+  int G;
   c->prof.begin(c->stream, KS_K_CGS);
-  k_cgs_update<T><<<G, kBlock, sizeof(double) * std::max(nv, 1), c->stream>>>(
-      n, V, ldv, nv, w, hcol, ldh, g, red(c), stat(c, KS_K_CGS, bytes));
+  if (vec_ok(V, ldv, (const T*)w)) {
+    G = grid_map(c, k_cgs_update<T, true>, (n + Vec16<T>::N - 1) / Vec16<T>::N, lds);
+    k_cgs_update<T, true><<<G, kBlock, lds, c->stream>>>(n, V, ldv, nv, w, hcol, ldh, g, red(c),
+                                                         stat(c, KS_K_CGS, bytes));
+  } else {
+    G = grid_map(c, k_cgs_update<T, false>, n, lds);
+    k_cgs_update<T, false><<<G, kBlock, lds, c->stream>>>(n, V, ldv, nv, w, hcol, ldh, g, red(c),
+                                                          stat(c, KS_K_CGS, bytes));
+  }
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
   launch_finish<1>(c, G, FinWrap<Fin>{f}, g);
@@ -759,10 +779,12 @@ struct GmresLevel : Level {
     const double bytes = (double)sizeof(T) * n * 2 + (inner ? 8.0 : (double)sizeof(T)) * n * (R - 1);
     c->prof.begin(c->stream, KS_K_CGS);
     if (inner)
-      k_assemble<T, double><<<grid_map(n), kBlock, sizeof(double) * R, c->stream>>>(
+      k_assemble<T, double><<<grid_map(c, k_assemble<T, double>, n, sizeof(double) * R), kBlock,
+                               sizeof(double) * R, c->stream>>>(
           n, Z.p, ldv, x, h, last, gdone(), stat(c, KS_K_CGS, bytes));
     else
-      k_assemble<T, T><<<grid_map(n), kBlock, sizeof(double) * R, c->stream>>>(
+      k_assemble<T, T><<<grid_map(c, k_assemble<T, T>, n, sizeof(double) * R), kBlock, sizeof(double) * R,
+                          c->stream>>>(
           n, V.p, ldv, x, h, last, gdone(), stat(c, KS_K_CGS, bytes));
     c->prof.end(c->stream);
     KS_HIPCK(hipGetLastError());
@@ -931,7 +953,8 @@ struct GcrLevel : Level {
     launch_proj<T>(c, n, Q.p, ldv, k + 1, Av.p, hctl.y, 1, g);
     const double bytes = (double)sizeof(T) * n * (2.0 * (k + 1) + 4.0);
     c->prof.begin(c->stream, KS_K_CGS);
-    k_gcr_update<T><<<grid_map(n), kBlock, sizeof(double) * (k + 1), c->stream>>>(
+    k_gcr_update<T><<<grid_map(c, k_gcr_update<T>, n, sizeof(double) * (k + 1)), kBlock, sizeof(double) * (k + 1),
+                      c->stream>>>(
         n, P.p, Q.p, ldv, k + 1, src, Av.p, hctl.y, hctl.e, P.p + (int64_t)(k + 1) * ldv,
         Q.p + (int64_t)(k + 1) * ldv, g, stat(c, KS_K_CGS, bytes));
     c->prof.end(c->stream);

@@ -21,8 +21,12 @@ def _mat(ctx, ptr, col, val, **kw):
     return api.Matrix(ctx, ptr, col, val, **kw)
 
 
+LAYOUTS = ["stream", "wg", "vector", "merge"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("case", ["poisson2d", "convdiff3d", "zipf"])
-def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case):
+def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     if case == "poisson2d":
         ptr, col, val, _, _ = P.poisson2d(40)
     elif case == "convdiff3d":
@@ -31,7 +35,8 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case):
         ptr, col, val, _ = P.zipf_small(5000, 600)
     n = len(ptr) - 1
     x = np.random.default_rng(7).standard_normal(n)
-    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
+    assert A.info()["spmv_kind"] == {"stream": 0, "wg": 0, "vector": 1, "merge": 2}[layout]
     m = np.diff(ptr).max()
     y = A.spmv(x)
     yo = oracle.spmv(ptr, col, val, x)
@@ -56,8 +61,9 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case):
     assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
 
 
+@pytest.mark.parametrize("layout", ["stream", "wg"])
 @pytest.mark.parametrize("case", ["poisson2d", "convdiff3d", "aniso"])
-def test_spmv_bitexact_short_rows(ctx, oracle, case):
+def test_spmv_bitexact_short_rows(ctx, oracle, case, layout):
     # rows of <= 8 entries are summed sequentially in CSR order from products
     # rounded in the value type: the oracle's (the reference's) exact order
     if case == "poisson2d":
@@ -68,13 +74,14 @@ def test_spmv_bitexact_short_rows(ctx, oracle, case):
         ptr, col, val, _ = P.aniso(37)
     n = len(ptr) - 1
     x = np.random.default_rng(11).standard_normal(n)
-    A = _mat(ctx, ptr, col, val, fp32=True)
+    A = _mat(ctx, ptr, col, val, fp32=True, layout=layout)
     assert np.array_equal(A.spmv(x), oracle.spmv(ptr, col, val, x))
     x32 = x.astype(np.float32)
     assert np.array_equal(A.spmv(x32, precision=32), oracle.spmv_f32(ptr, col, val.astype(np.float32), x32))
 
 
-def test_spmv_edge_cases(ctx, oracle):
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_spmv_edge_cases(ctx, oracle, layout):
     # empty rows, a dense row, a single entry
     n = 70
     rows = [[], [0], list(range(n)), [5, 69], []] + [[i] for i in range(5, n)]
@@ -85,7 +92,7 @@ def test_spmv_edge_cases(ctx, oracle):
         ptr[i + 1] = len(cols)
     col = np.array(cols, np.int32)
     val = np.random.default_rng(3).standard_normal(len(col))
-    A = _mat(ctx, ptr, col, val, transpose=True)
+    A = _mat(ctx, ptr, col, val, transpose=True, layout=layout)
     x = np.random.default_rng(4).standard_normal(n)
     y = A.spmv(x)
     yo = oracle.spmv(ptr, col, val, x)
@@ -96,13 +103,15 @@ def test_spmv_edge_cases(ctx, oracle):
     assert np.array_equal(tp, otp) and np.array_equal(tc, otc) and np.array_equal(tv, otv)
 
 
-def test_spmv_long_rows(ctx, oracle):
-    # Zipf rows up to 4096 entries: rows longer than one LDS tile (2048)
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_spmv_long_rows(ctx, oracle, layout):
+    # Zipf rows up to 4096 entries: rows longer than a workgroup tile (2048),
+    # a wavefront tile (512) and a merge-path segment (512 items)
     ptr, col, val, _ = P.zipf_small(20_000, 4096)
     assert np.diff(ptr).max() > 2048
     n = len(ptr) - 1
     x = np.random.default_rng(9).standard_normal(n)
-    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
     m = np.diff(ptr).max()
     y = A.spmv(x)
     yo = oracle.spmv(ptr, col, val, x)
@@ -111,6 +120,36 @@ def test_spmv_long_rows(ctx, oracle):
     otp, otc, otv = oracle.transpose(ptr, col, val)
     yto = oracle.spmv(otp, otc, otv, x)
     assert np.all(np.abs(yt - yto) <= 4 * np.diff(otp).max() * EPS64 * P.row_abs_bound(otp, otc, otv, x) + 1e-300)
+    x32 = x.astype(np.float32)
+    v32 = val.astype(np.float32)
+    y32 = A.spmv(x32, precision=32)
+    yo32 = oracle.spmv_f32(ptr, col, v32, x32)
+    b32 = 4 * m * EPS32 * P.row_abs_bound(ptr, col, v32.astype(np.float64), x32) + 1e-30
+    assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
+
+
+def test_borrowed_matrix(ctx, oracle):
+    # KS_MAT_BORROW: the caller's device CSR is used in place
+    import torch
+    from solver_cpp_amd import api
+    ptr, col, val, _, _ = P.convdiff3d(12)
+    n = len(ptr) - 1
+    tp, tc, tv = (torch.from_numpy(a).cuda() for a in (ptr, col, val))
+    A = _mat(ctx, tp, tc, tv, transpose=True, fp32=True, borrow=True)
+    x = np.random.default_rng(3).standard_normal(n)
+    assert np.array_equal(A.spmv(x), oracle.spmv(ptr, col, val, x))
+    # values changed in place, then made visible to the derived copies
+    tv.mul_(2.0)
+    A.update_values(tv)
+    v2 = 2.0 * val
+    assert np.array_equal(A.spmv(x), oracle.spmv(ptr, col, v2, x))
+    x32 = x.astype(np.float32)
+    assert np.array_equal(A.spmv(x32, precision=32), oracle.spmv_f32(ptr, col, v2.astype(np.float32), x32))
+    otp, otc, otv = oracle.transpose(ptr, col, v2)
+    assert np.array_equal(A.spmv(x, trans=True), oracle.spmv(otp, otc, otv, x))
+    A.close()
+    with pytest.raises(api.KsolveError):
+        api.Matrix(ctx, ptr, col, val, borrow=True)  # host arrays cannot be borrowed
 
 
 def test_bad_csr_rejected(ctx):
@@ -150,9 +189,9 @@ def test_deterministic_reductions(ctx):
     assert len(vals) == 1
 
 
+@pytest.mark.parametrize("n", [50_000, 50_001])  # 16-byte vector path / scalar path
 @pytest.mark.parametrize("k", [0, 3, 20, 40])
-def test_cgs_step(ctx, oracle, k):
-    n = 50_000
+def test_cgs_step(ctx, oracle, k, n):
     rng = np.random.default_rng(k)
     V = np.linalg.qr(rng.standard_normal((n, k + 1)))[0].T.copy()
     w = rng.standard_normal(n)
@@ -164,13 +203,14 @@ def test_cgs_step(ctx, oracle, k):
     assert np.abs(V @ wg).max() <= 1e-12 * np.linalg.norm(w)
 
 
+@pytest.mark.parametrize("layout", ["stream", "wg", "merge"])
 @pytest.mark.parametrize("k", [1, 2, 4])
-def test_kskip_basis_and_gram(ctx, oracle, k):
+def test_kskip_basis_and_gram(ctx, oracle, k, layout):
     ptr, col, val, _, _ = P.convdiff3d(10, pe=0.0)
     n = len(ptr) - 1
     rng = np.random.default_rng(k)
     r, p = rng.standard_normal(n), rng.standard_normal(n)
-    A = _mat(ctx, ptr, col, val)
+    A = _mat(ctx, ptr, col, val, layout=layout)
     Ar, Ap = A.kskip_basis(k, r, p)
     Aro, Apo = oracle.kskip_basis(ptr, col, val, k, r, p)
     for G, O in ((Ar, Aro), (Ap, Apo)):

@@ -201,3 +201,27 @@ def test_adjoint_small(ctx, oracle):
     assert np.linalg.norm(gb - lam) <= 1e-9 * np.linalg.norm(lam)
     rows = np.repeat(np.arange(n), np.diff(ptr))
     assert np.allclose(gv, -lam[rows] * x[col], rtol=0, atol=1e-9 * np.abs(lam).max() * np.abs(x).max())
+
+
+@pytest.mark.parametrize("layout", ["wg", "vector", "merge"])
+def test_solvers_on_every_spmv_layout(ctx, oracle, layout):
+    # the fused SpMV epilogues (residual + norm, dot) and the two-input
+    # k-skip basis on each SpMV layout give the default layout's results
+    from solver_cpp_amd import api
+    cases = [("cg", "NO", P.poisson2d(30)[:4], {}),
+             ("gmres", "NO", P.convdiff3d(11)[:4], dict(outer_restart=11)),
+             ("bicg", "NO", P.convdiff3d(11)[:4], {}),
+             ("kskipcg", "NO", P.poisson2d(30)[:4], dict(outer_kskip=2, outer_fix=2)),
+             ("vpgmres", "gmres", P.zipf_small(3000, 32)[:4], dict(inner_precision=32, inner_maxloop=11,
+                                                                 inner_restart=12, outer_restart=21))]
+    for outer, inner, (ptr, col, val, b), kw in cases:
+        kw = dict(outer_eps=1e-10, outer_maxloop=3000, **kw)
+        A0 = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+        A1 = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
+        x0, r0, _ = api.solve(ctx, A0, b, outer, inner, **kw)
+        x1, r1, _ = api.solve(ctx, A1, b, outer, inner, **kw)
+        assert r0["status"] == 0 and r1["status"] == 0, (outer, r0, r1)
+        assert abs(r0["iters"] - r1["iters"]) <= 1, (outer, r0["iters"], r1["iters"])
+        assert np.linalg.norm(x1 - x0) <= 1e-7 * np.linalg.norm(x0), outer
+        xo, ro, _ = oracle.solve(ptr, col, val, b, outer, inner, **kw)
+        assert abs(r1["iters"] - ro["iters"]) <= 1, (outer, r1["iters"], ro["iters"])
