@@ -265,6 +265,23 @@ ks_status ks_crs_write(const char* dir, int64_t n, int64_t nnz, const int32_t* p
                        const int32_t* col, const double* val, const double* b,
                        const double* x);
 
+/* ---- binary CSR sidecar of the CRS text files ----------------------------
+ * No reference counterpart: a versioned binary copy of ColVal/Ptr/bx (the
+ * Solver tool keeps it as <dir>/CRS.ksb with --bin-cache) that loads at
+ * file-read speed instead of parsing text (collection::readMatrix,
+ * solver_collection.cpp:726-828, fscanf per line).  The header records a
+ * stamp of the text files it was made from (ks_crs_stamp: sizes and
+ * modification times) and a checksum of the arrays; ks_crsbin_read rejects a
+ * file whose size or checksum does not match (KS_ERR_IO). */
+ks_status ks_crs_stamp(const char* dir, uint64_t* stamp);
+ks_status ks_crsbin_write(const char* path, int64_t n, int64_t nnz, const int32_t* ptr,
+                          const int32_t* col, const double* val, const double* b,
+                          const double* x, uint64_t stamp);
+ks_status ks_crsbin_read_header(const char* path, int64_t* n, int64_t* nnz,
+                                uint64_t* stamp);
+ks_status ks_crsbin_read(const char* path, int32_t* ptr, int32_t* col, double* val,
+                         double* b, double* x);
+
 #ifdef __cplusplus
 }
 #endif

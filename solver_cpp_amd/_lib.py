@@ -52,6 +52,10 @@ _SIGS = {
     "ks_crs_read_header": (_i32, [C.c_char_p, C.POINTER(_i64), C.POINTER(_i64)]),
     "ks_crs_read": (_i32, [C.c_char_p, _vp, _vp, _vp, _vp, _vp]),
     "ks_crs_write": (_i32, [C.c_char_p, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "ks_crs_stamp": (_i32, [C.c_char_p, C.POINTER(_u64)]),
+    "ks_crsbin_write": (_i32, [C.c_char_p, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _u64]),
+    "ks_crsbin_read_header": (_i32, [C.c_char_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_u64)]),
+    "ks_crsbin_read": (_i32, [C.c_char_p, _vp, _vp, _vp, _vp, _vp]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -6,7 +6,9 @@
 // `not converge`.  All solving runs on one GPU through libksolve.
 //
 // Added flags (no reference counterpart): --inner-precision 32|64,
-// --ref-compat BITS, --device N, --json (one JSON result line).
+// --ref-compat BITS, --device N, --json (one JSON result line), --bin-cache
+// (load <dir>/CRS.ksb, the binary copy of the text files, when it is current;
+// write it otherwise).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -71,6 +73,7 @@ int main(int argc, char** argv) {
       {"pinned", 'x', false, ""},             {"multiGPU", 'g', false, ""},
       {"inner-precision", 0, true, "64"},     {"ref-compat", 0, true, "0"},
       {"device", 0, true, "0"},               {"json", 0, false, ""},
+      {"bin-cache", 0, false, ""},
       {"help", 'h', false, ""},
   };
   std::map<std::string, Opt*> bylong;
@@ -165,13 +168,33 @@ int main(int argc, char** argv) {
   }
   std::vector<int32_t> ptr(n + 1), col(nnz);
   std::vector<double> val(nnz), b(n), x(n);
-  if (ks_crs_read(full.c_str(), ptr.data(), col.data(), val.data(), b.data(), x.data()) != KS_OK) {
-    std::cerr << "[X] Read CRS files fail" << std::endl;
-    return 1;
+  // --bin-cache: <dir>/CRS.ksb is used when its stamp matches the text
+  // files, and (re)written from the text otherwise
+  bool from_cache = false;
+  const bool use_cache = bylong["bin-cache"]->set;
+  const std::string cache = full + "/CRS.ksb";
+  uint64_t stamp = 0;
+  if (use_cache && ks_crs_stamp(full.c_str(), &stamp) == KS_OK) {
+    int64_t cn = 0, cnnz = 0;
+    uint64_t cst = 0;
+    if (ks_crsbin_read_header(cache.c_str(), &cn, &cnnz, &cst) == KS_OK && cn == n && cnnz == nnz &&
+        cst == stamp)
+      from_cache = ks_crsbin_read(cache.c_str(), ptr.data(), col.data(), val.data(), b.data(), x.data()) == KS_OK;
+  }
+  if (!from_cache) {
+    if (ks_crs_read(full.c_str(), ptr.data(), col.data(), val.data(), b.data(), x.data()) != KS_OK) {
+      std::cerr << "[X] Read CRS files fail" << std::endl;
+      return 1;
+    }
+    if (use_cache && stamp &&
+        ks_crsbin_write(cache.c_str(), n, nnz, ptr.data(), col.data(), val.data(), b.data(), x.data(), stamp) !=
+            KS_OK)
+      std::cerr << "[!] could not write the binary cache " << cache << std::endl;
   }
   if (!is_dir("./output")) { std::cerr << "File open error" << std::endl; return 1; }
   std::cout << "=========================================================" << std::endl;
-  std::cout << "InputSource : CRS" << std::endl << "FullPath : " << full << std::endl;
+  std::cout << "InputSource : CRS" << (from_cache ? " (binary cache)" : "") << std::endl
+            << "FullPath : " << full << std::endl;
   std::cout << "N : " << n << std::endl << "NNZ : " << nnz << std::endl;
   std::cout << "OuterSolver : " << kUpper[outer] << std::endl;
   std::cout << "InnerSolver : " << (inner ? kUpper[inner] : "NONE") << std::endl;

@@ -12,6 +12,8 @@
 #include <thread>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "../../include/ksolve.h"
 
 namespace {
@@ -291,6 +293,8 @@ ks_status ks_gen_zipf(int64_t n, double s, int32_t cap, uint64_t seed_len, uint6
   return KS_OK;
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------- CRS text ----
 namespace {
 bool read_all(const std::string& path, std::string& out) {
@@ -314,7 +318,151 @@ bool header(const std::string& path, int64_t h[3]) {
   h[0] = a; h[1] = b; h[2] = c;
   return true;
 }
+
+unsigned io_threads(size_t bytes) {
+  if (bytes < (1u << 20)) return 1;
+  return std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
+inline bool is_ws(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+// Parses the first `ntok` whitespace-separated tokens of [b, e) with
+// fn(token_index, p, &end) -> ok, in parallel: the text is cut at token
+// starts, one pass counts each part's tokens, a second parses them at their
+// global indices.  Any whitespace layout is accepted (as fscanf would); a
+// token not followed by whitespace or the end of the text is an error.
+template <class Fn>
+bool parse_tokens(const char* b, const char* e, int64_t ntok, Fn fn) {
+  const unsigned T = io_threads((size_t)(e - b));
+  std::vector<const char*> cut(T + 1);
+  cut[0] = b;
+  cut[T] = e;
+  for (unsigned t = 1; t < T; ++t) {
+    const char* p = b + (e - b) * (int64_t)t / T;
+    while (p < e && p > b && !is_ws(p[-1])) ++p;
+    while (p < e && is_ws(*p)) ++p;
+    cut[t] = std::max(p, cut[t - 1]);
+  }
+  std::vector<int64_t> cnt(T + 1, 0);
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t)
+      th.emplace_back([&, t]() {
+        int64_t c = 0;
+        for (const char* q = cut[t]; q < cut[t + 1]; ++q)
+          if (!is_ws(*q) && (q == cut[t] || is_ws(q[-1]))) ++c;
+        cnt[t + 1] = c;
+      });
+    for (auto& x : th) x.join();
+  }
+  for (unsigned t = 0; t < T; ++t) cnt[t + 1] += cnt[t];
+  if (cnt[T] < ntok) return false;
+  std::vector<char> ok(T, 1);
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t)
+      th.emplace_back([&, t]() {
+        int64_t idx = cnt[t];
+        const char* q = cut[t];
+        const char* lim = cut[t + 1];
+        while (idx < ntok) {
+          while (q < lim && is_ws(*q)) ++q;
+          if (q >= lim) break;
+          char* end = nullptr;
+          if (!fn(idx, q, &end) || end == q || (end < e && !is_ws(*end))) { ok[t] = 0; return; }
+          q = end;
+          ++idx;
+        }
+      });
+    for (auto& x : th) x.join();
+  }
+  for (char v : ok)
+    if (!v) return false;
+  return true;
+}
+
+// the text after the three header integers
+const char* after_header(const std::string& buf) {
+  const char* p = buf.c_str();
+  char* end = nullptr;
+  for (int i = 0; i < 3; ++i) {
+    std::strtoll(p, &end, 10);
+    p = end;
+  }
+  return p;
+}
+
+// ------------------------------------------------ binary CSR sidecar ----
+// Layout (little-endian): a 64-byte header, then ptr (int32, n+1), col
+// (int32, nnz), val (f64, nnz), b (f64, n) and x (f64, n), each section
+// starting on an 8-byte boundary.  The checksum covers the sections.
+struct KsbHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t flags;  // 1: b present, 2: x present
+  int64_t n, nnz;
+  uint64_t stamp;     // fingerprint of the CRS text files it was made from
+  uint64_t checksum;  // of the payload sections
+  uint64_t reserved[2];
+};
+static_assert(sizeof(KsbHeader) == 64, "header is 64 bytes");
+constexpr char kKsbMagic[8] = {'K', 'S', 'C', 'S', 'R', 'B', 'I', 'N'};
+constexpr uint32_t kKsbVersion = 1;
+
+inline uint64_t mix64(uint64_t z) {  // SplitMix64 finaliser
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// Hash of one section: 1 MiB blocks hashed word by word (in parallel), the
+// block hashes folded in block order - the value does not depend on the
+// thread count.
+uint64_t section_hash(const void* data, size_t len) {
+  constexpr size_t kBlk = 1 << 20;
+  const size_t nb = (len + kBlk - 1) / kBlk;
+  std::vector<uint64_t> hb(nb);
+  const unsigned T = std::max<unsigned>(1, std::min<size_t>(io_threads(len), nb));
+  auto work = [&](unsigned t) {
+    for (size_t i = t; i < nb; i += T) {
+      const unsigned char* p = (const unsigned char*)data + i * kBlk;
+      const size_t m = std::min(kBlk, len - i * kBlk);
+      uint64_t h = mix64(0x6b73637372ULL + i);
+      size_t j = 0;
+      for (; j + 8 <= m; j += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + j, 8);
+        h = mix64(h ^ w);
+      }
+      if (j < m) {
+        uint64_t w = 0;
+        std::memcpy(&w, p + j, m - j);
+        h = mix64(h ^ w ^ ((uint64_t)(m - j) << 56));
+      }
+      hb[i] = h;
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < T; ++t) th.emplace_back(work, t);
+  for (auto& x : th) x.join();
+  uint64_t h = mix64(len);
+  for (uint64_t v : hb) h = mix64(h ^ v);
+  return h;
+}
+
+struct Section {
+  const void* p;
+  size_t len;
+};
+uint64_t payload_hash(const Section* s, int k) {
+  uint64_t h = 0x5eedULL;
+  for (int i = 0; i < k; ++i) h = mix64(h ^ section_hash(s[i].p, s[i].len) ^ (uint64_t)i);
+  return h;
+}
+inline size_t pad8(size_t b) { return (8 - b % 8) % 8; }
 }  // namespace
+
+extern "C" {
 
 // collection::checkCRSMatrix (solver_collection.cpp:655-724): the three
 // headers must agree on N, M, NNZ and N must equal M.
@@ -333,53 +481,48 @@ ks_status ks_crs_read_header(const char* dir, int64_t* n, int64_t* nnz) {
   return KS_OK;
 }
 
-// collection::readMatrix (solver_collection.cpp:726-828)
+// collection::readMatrix (solver_collection.cpp:726-828): ColVal.txt holds
+// nnz "col val" records, Ptr.txt n+1 row pointers, bx.txt n "b x" records,
+// each after the "N M NNZ" header.  Parsed in parallel (parse_tokens).
 ks_status ks_crs_read(const char* dir, int32_t* ptr, int32_t* col, double* val, double* b, double* x) {
   int64_t n, nnz;
   ks_status st = ks_crs_read_header(dir, &n, &nnz);
   if (st != KS_OK) return st;
   const std::string d(dir);
   std::string buf;
-  auto skip_header = [](const char*& p, char** end) {
-    for (int i = 0; i < 3; ++i) { std::strtoll(p, end, 10); p = *end; }
-  };
-  char* end = nullptr;
   if (!read_all(d + "/ColVal.txt", buf)) return KS_ERR_IO;
-  const char* p = buf.c_str();
-  skip_header(p, &end);
-  for (int64_t z = 0; z < nnz; ++z) {
-    long c = std::strtol(p, &end, 10);
-    if (end == p) return KS_ERR_IO;
-    p = end;
-    double v = std::strtod(p, &end);
-    if (end == p) return KS_ERR_IO;
-    p = end;
-    if (col) col[z] = (int32_t)c;
-    if (val) val[z] = v;
-  }
+  const char* e = buf.c_str() + buf.size();
+  bool ok = parse_tokens(after_header(buf), e, 2 * nnz, [&](int64_t i, const char* p, char** end) {
+    if (i % 2 == 0) {
+      const long c = std::strtol(p, end, 10);
+      if (col) col[i / 2] = (int32_t)c;
+    } else {
+      const double v = std::strtod(p, end);
+      if (val) val[i / 2] = v;
+    }
+    return true;
+  });
+  if (!ok) return KS_ERR_IO;
   if (!read_all(d + "/Ptr.txt", buf)) return KS_ERR_IO;
-  p = buf.c_str();
-  skip_header(p, &end);
-  for (int64_t z = 0; z < n + 1; ++z) {
-    long c = std::strtol(p, &end, 10);
-    if (end == p) return KS_ERR_IO;
-    p = end;
-    if (ptr) ptr[z] = (int32_t)c;
-  }
+  e = buf.c_str() + buf.size();
+  ok = parse_tokens(after_header(buf), e, n + 1, [&](int64_t i, const char* p, char** end) {
+    const long c = std::strtol(p, end, 10);
+    if (ptr) ptr[i] = (int32_t)c;
+    return true;
+  });
+  if (!ok) return KS_ERR_IO;
   if (!read_all(d + "/bx.txt", buf)) return KS_ERR_IO;
-  p = buf.c_str();
-  skip_header(p, &end);
-  for (int64_t z = 0; z < n; ++z) {
-    double v1 = std::strtod(p, &end);
-    if (end == p) return KS_ERR_IO;
-    p = end;
-    double v2 = std::strtod(p, &end);
-    if (end == p) return KS_ERR_IO;
-    p = end;
-    if (b) b[z] = v1;
-    if (x) x[z] = v2;
-  }
-  return KS_OK;
+  e = buf.c_str() + buf.size();
+  ok = parse_tokens(after_header(buf), e, 2 * n, [&](int64_t i, const char* p, char** end) {
+    const double v = std::strtod(p, end);
+    if (i % 2 == 0) {
+      if (b) b[i / 2] = v;
+    } else if (x) {
+      x[i / 2] = v;
+    }
+    return true;
+  });
+  return ok ? KS_OK : KS_ERR_IO;
 }
 
 ks_status ks_crs_write(const char* dir, int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col,
@@ -401,6 +544,131 @@ ks_status ks_crs_write(const char* dir, int64_t n, int64_t nnz, const int32_t* p
   std::fprintf(f, "%lld %lld %lld\n", (long long)n, (long long)n, (long long)nnz);
   for (int64_t z = 0; z < n; ++z) std::fprintf(f, "%.17e %.17e\n", b ? b[z] : 0.0, x ? x[z] : 0.0);
   std::fclose(f);
+  return KS_OK;
+}
+
+// Fingerprint (size and modification time) of the three CRS text files.
+ks_status ks_crs_stamp(const char* dir, uint64_t* stamp) {
+  if (!dir || !stamp) return KS_ERR_BAD_INPUT;
+  const std::string d(dir);
+  uint64_t h = 0x63727374616d70ULL;
+  for (const char* nm : {"/ColVal.txt", "/Ptr.txt", "/bx.txt"}) {
+    struct stat sb;
+    if (::stat((d + nm).c_str(), &sb) != 0) return KS_ERR_IO;
+    h = mix64(h ^ (uint64_t)sb.st_size);
+    h = mix64(h ^ (uint64_t)sb.st_mtim.tv_sec);
+    h = mix64(h ^ (uint64_t)sb.st_mtim.tv_nsec);
+  }
+  *stamp = h ? h : 1;
+  return KS_OK;
+}
+
+ks_status ks_crsbin_write(const char* path, int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col,
+                          const double* val, const double* b, const double* x, uint64_t stamp) {
+  if (!path || n < 1 || nnz < 0 || !ptr || (nnz > 0 && (!col || !val))) return KS_ERR_BAD_INPUT;
+  KsbHeader h;
+  std::memset(&h, 0, sizeof h);
+  std::memcpy(h.magic, kKsbMagic, 8);
+  h.version = kKsbVersion;
+  h.flags = (b ? 1u : 0u) | (x ? 2u : 0u);
+  h.n = n;
+  h.nnz = nnz;
+  h.stamp = stamp;
+  Section sec[5] = {{ptr, sizeof(int32_t) * (size_t)(n + 1)}, {col, sizeof(int32_t) * (size_t)nnz},
+                    {val, sizeof(double) * (size_t)nnz}, {b, b ? sizeof(double) * (size_t)n : 0},
+                    {x, x ? sizeof(double) * (size_t)n : 0}};
+  h.checksum = payload_hash(sec, 5);
+  // written to a temporary name, then renamed: a reader never sees a partial file
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return KS_ERR_IO;
+  static const char zeros[8] = {0};
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  for (int i = 0; i < 5 && ok; ++i) {
+    if (sec[i].len) ok = std::fwrite(sec[i].p, 1, sec[i].len, f) == sec[i].len;
+    const size_t pd = pad8(sec[i].len);
+    if (ok && pd) ok = std::fwrite(zeros, 1, pd, f) == pd;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return KS_ERR_IO;
+  }
+  return KS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+ks_status ksb_header(FILE* f, KsbHeader& h) {
+  if (std::fread(&h, sizeof h, 1, f) != 1) return KS_ERR_IO;
+  if (std::memcmp(h.magic, kKsbMagic, 8) != 0 || h.version != kKsbVersion) return KS_ERR_IO;
+  if (h.n < 1 || h.n >= (int64_t)INT32_MAX || h.nnz < 0 || h.nnz >= (int64_t)INT32_MAX) return KS_ERR_IO;
+  return KS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+ks_status ks_crsbin_read_header(const char* path, int64_t* n, int64_t* nnz, uint64_t* stamp) {
+  if (!path) return KS_ERR_BAD_INPUT;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return KS_ERR_IO;
+  KsbHeader h;
+  const ks_status st = ksb_header(f, h);
+  std::fclose(f);
+  if (st != KS_OK) return st;
+  if (n) *n = h.n;
+  if (nnz) *nnz = h.nnz;
+  if (stamp) *stamp = h.stamp;
+  return KS_OK;
+}
+
+// Reads the sections into the given arrays (NULL: read and checked, not
+// returned); b or x absent from the file read as zeros.  The file size and
+// the checksum must match the header.
+ks_status ks_crsbin_read(const char* path, int32_t* ptr, int32_t* col, double* val, double* b, double* x) {
+  if (!path) return KS_ERR_BAD_INPUT;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return KS_ERR_IO;
+  KsbHeader h;
+  ks_status st = ksb_header(f, h);
+  if (st != KS_OK) {
+    std::fclose(f);
+    return st;
+  }
+  const size_t n = (size_t)h.n, nnz = (size_t)h.nnz;
+  const bool hb = h.flags & 1u, hx = h.flags & 2u;
+  size_t len[5] = {4 * (n + 1), 4 * nnz, 8 * nnz, hb ? 8 * n : 0, hx ? 8 * n : 0};
+  size_t expect = sizeof h;
+  for (size_t l : len) expect += l + pad8(l);
+  std::fseek(f, 0, SEEK_END);
+  const long sz = std::ftell(f);
+  if (sz < 0 || (size_t)sz != expect) {
+    std::fclose(f);
+    return KS_ERR_IO;
+  }
+  std::fseek(f, sizeof h, SEEK_SET);
+  void* dst[5] = {ptr, col, val, b, x};
+  std::vector<std::vector<char>> tmp(5);
+  Section sec[5];
+  bool ok = true;
+  for (int i = 0; i < 5 && ok; ++i) {
+    void* p = dst[i];
+    if (!p || !len[i]) {
+      tmp[i].resize(len[i]);
+      p = tmp[i].data();
+    }
+    if (len[i]) ok = std::fread(p, 1, len[i], f) == len[i];
+    char padb[8];
+    const size_t pd = pad8(len[i]);
+    if (ok && pd) ok = std::fread(padb, 1, pd, f) == pd;
+    sec[i] = Section{p, len[i]};
+  }
+  std::fclose(f);
+  if (!ok || payload_hash(sec, 5) != h.checksum) return KS_ERR_IO;
+  if (b && !hb) std::fill(b, b + n, 0.0);
+  if (x && !hx) std::fill(x, x + n, 0.0);
   return KS_OK;
 }
 

@@ -116,6 +116,41 @@ def read_crs(path):
     return ptr, col, val, b, x
 
 
+def crs_stamp(path):
+    """Fingerprint of a CRS text directory (sizes, modification times)."""
+    st = C.c_uint64()
+    _chk(load().ks_crs_stamp(path.encode(), C.byref(st)), "ks_crs_stamp")
+    return st.value
+
+
+def write_crs_bin(file, ptr, col, val, b=None, x=None, stamp=0):
+    """Write the binary CSR sidecar (ks_crsbin_write)."""
+    n = len(ptr) - 1
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    bp = None if b is None else np.ascontiguousarray(b, np.float64)
+    xp = None if x is None else np.ascontiguousarray(x, np.float64)
+    _chk(load().ks_crsbin_write(file.encode(), n, len(col), ptr.ctypes.data, col.ctypes.data, val.ctypes.data,
+                                None if bp is None else bp.ctypes.data, None if xp is None else xp.ctypes.data,
+                                stamp), "ks_crsbin_write")
+
+
+def read_crs_bin(file):
+    """Read the binary CSR sidecar: (ptr, col, val, b, x, stamp)."""
+    L = load()
+    n, nnz, st = C.c_int64(), C.c_int64(), C.c_uint64()
+    _chk(L.ks_crsbin_read_header(file.encode(), C.byref(n), C.byref(nnz), C.byref(st)), "ks_crsbin_read_header")
+    ptr = np.empty(n.value + 1, np.int32)
+    col = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float64)
+    b = np.empty(n.value, np.float64)
+    x = np.empty(n.value, np.float64)
+    _chk(L.ks_crsbin_read(file.encode(), ptr.ctypes.data, col.ctypes.data, val.ctypes.data, b.ctypes.data,
+                          x.ctypes.data), "ks_crsbin_read")
+    return ptr, col, val, b, x, st.value
+
+
 # --- benchmark problems (seeds fixed by the benchmark plan) ---------------
 def problem(cfg, scale=None):
     """Matrix and right-hand side of a benchmark configuration.

@@ -225,3 +225,34 @@ def test_solvers_on_every_spmv_layout(ctx, oracle, layout):
         assert np.linalg.norm(x1 - x0) <= 1e-7 * np.linalg.norm(x0), outer
         xo, ro, _ = oracle.solve(ptr, col, val, b, outer, inner, **kw)
         assert abs(r1["iters"] - ro["iters"]) <= 1, (outer, r1["iters"], ro["iters"])
+
+
+def test_cli_solver_end_to_end(tmp_path, oracle):
+    # the Solver tool on CRS text files (and then on its binary cache) gives
+    # the library's result: same loop count as the oracle, same x files
+    import os
+    import re
+    import subprocess
+    from solver_cpp_amd import gen
+    solver = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "solver_cpp_amd", "Solver")
+    ptr, col, val, b, _ = P.convdiff3d(10)
+    root = tmp_path / "root"
+    gen.write_crs(str(root / "Matrix" / "CRS" / "CD10"), ptr, col, val, b)
+    work = tmp_path / "work"
+    (work / "output").mkdir(parents=True)
+    args = [solver, "-S", "gmres", "-R", "11", "-E", "1e-10", "-L", "2000", "-m", "CD10", "-p", str(root),
+            "--bin-cache"]
+    xo, ro, ho = oracle.solve(ptr, col, val, b, "gmres", outer_restart=11, outer_eps=1e-10, outer_maxloop=2000)
+    outs = []
+    for source in ("CRS\n", "CRS (binary cache)"):
+        r = subprocess.run(args, cwd=str(work), capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "InputSource : " + source in r.stdout and r.stdout.rstrip().endswith("converge")
+        loop = int(re.search(r"^loop = (\d+)$", r.stdout, re.M).group(1))
+        assert abs(loop - ro["iters"]) <= 1
+        xv = np.loadtxt(work / "output" / "GMRES_xvec.txt")[:, 1]
+        his = np.loadtxt(work / "output" / "GMRES_his.txt")
+        assert np.array_equal(his[:, 0], ho[:, 0])  # same check indices as the oracle
+        assert np.linalg.norm(xv - xo) <= 1e-8 * np.linalg.norm(xo)
+        outs.append(xv)
+    assert np.array_equal(outs[0], outs[1])

@@ -191,6 +191,72 @@ def test_crs_roundtrip_and_header_checks(tmp_path):
         gen.read_crs(d)
 
 
+def _write_messy_crs(d, ptr, col, val, b, x):
+    # the reference's fscanf accepts any whitespace: tabs, runs of blanks,
+    # CRLF line ends and several records on one line
+    os.makedirs(d, exist_ok=True)
+    n, nnz = len(ptr) - 1, len(col)
+    hdr = f"{n}  {n}\t{nnz}\r\n"
+    recs = [f"{c}\t {float(v)!r}" for c, v in zip(col, val)]
+    body = "".join(r + ("  " if i % 3 else "\r\n") for i, r in enumerate(recs))
+    open(os.path.join(d, "ColVal.txt"), "w", newline="").write(hdr + body + "\n")
+    open(os.path.join(d, "Ptr.txt"), "w", newline="").write(hdr + " ".join(str(int(p)) for p in ptr) + "\n")
+    bx = "\n".join(f"{float(bi)!r}   {float(xi)!r}" for bi, xi in zip(b, x))
+    open(os.path.join(d, "bx.txt"), "w", newline="").write(hdr + bx + "\n\n")
+
+
+def test_crs_parallel_parse_any_whitespace(tmp_path):
+    # > 1 MiB of text, so the parser splits it across threads
+    ptr, col, val = gen.stencil(3, 24, 24, 24, pe=0.5)
+    n = len(ptr) - 1
+    b = gen.vector(7, n, "uniform")
+    x = gen.vector(8, n, "normal")
+    d = str(tmp_path / "messy")
+    _write_messy_crs(d, ptr, col, val, b, x)
+    assert os.path.getsize(os.path.join(d, "ColVal.txt")) > (1 << 20)
+    p2, c2, v2, b2, x2 = gen.read_crs(d)
+    # repr() round-trips exactly, so the parse must be bit-exact
+    assert np.array_equal(p2, ptr) and np.array_equal(c2, col) and np.array_equal(v2, val)
+    assert np.array_equal(b2, b) and np.array_equal(x2, x)
+    # a malformed token is an error, not a silent truncation
+    hdr, body = open(os.path.join(d, "Ptr.txt")).read().split("\n", 1)
+    toks = body.split(" ")
+    toks[5] += "x"
+    open(os.path.join(d, "Ptr.txt"), "w").write(hdr + "\n" + " ".join(toks))
+    with pytest.raises(RuntimeError):
+        gen.read_crs(d)
+
+
+def test_crs_binary_sidecar(tmp_path):
+    ptr, col, val = gen.stencil(3, 9, 8, 7, pe=0.8)
+    n = len(ptr) - 1
+    b = gen.vector(3, n, "uniform")
+    x = gen.vector(4, n, "normal")
+    d = str(tmp_path / "M")
+    gen.write_crs(d, ptr, col, val, b, x)
+    st = gen.crs_stamp(d)
+    f = os.path.join(d, "CRS.ksb")
+    gen.write_crs_bin(f, ptr, col, val, b, x, stamp=st)
+    p2, c2, v2, b2, x2, st2 = gen.read_crs_bin(f)
+    assert st2 == st
+    for u, v in ((p2, ptr), (c2, col), (v2, val), (b2, b), (x2, x)):
+        assert np.array_equal(u, v)
+    # the stamp follows the text files
+    txt = open(os.path.join(d, "bx.txt")).read()
+    open(os.path.join(d, "bx.txt"), "w").write(txt + "\n")
+    assert gen.crs_stamp(d) != st
+    # one flipped payload byte, a truncated file, a foreign header: all rejected
+    raw = bytearray(open(f, "rb").read())
+    for bad in (raw[:64] + raw[64:80] + bytes([raw[80] ^ 1]) + raw[81:], raw[:-8], b"NOTKSCSR" + raw[8:]):
+        open(f, "wb").write(bytes(bad))
+        with pytest.raises(RuntimeError):
+            gen.read_crs_bin(f)
+    # without b / x: read as zeros
+    gen.write_crs_bin(f, ptr, col, val)
+    p3, c3, v3, b3, x3, st3 = gen.read_crs_bin(f)
+    assert st3 == 0 and np.array_equal(v3, val) and not b3.any() and not x3.any()
+
+
 # --------------------------------------------------------------- the CLI --
 SOLVER = os.path.join(ROOT, "solver_cpp_amd", "Solver")
 
@@ -217,3 +283,23 @@ def test_cli_flag_errors(tmp_path):
     assert r.returncode != 0
     r = _cli("-h")
     assert r.returncode == 0 and "OuterSolver" in r.stderr
+
+
+def test_cli_bin_cache_written_then_used(tmp_path):
+    # the matrix is read (and the cache written) before the GPU is opened, so
+    # this runs without a GPU: the solve itself then fails with "no GPU"
+    ptr, col, val = gen.stencil(2, 12, 12, 1, pe=0.0)
+    n = len(ptr) - 1
+    root = tmp_path / "root"
+    d = root / "Matrix" / "CRS" / "P12"
+    gen.write_crs(str(d), ptr, col, val, gen.vector(1, n, "uniform"))
+    work = tmp_path / "work"
+    (work / "output").mkdir(parents=True)
+    args = ("-S", "cg", "-m", "P12", "-p", str(root), "--bin-cache")
+    r1 = _cli(*args, cwd=str(work))
+    assert os.path.exists(d / "CRS.ksb")
+    assert "InputSource : CRS\n" in r1.stdout
+    r2 = _cli(*args, cwd=str(work))
+    assert "InputSource : CRS (binary cache)" in r2.stdout
+    p2, c2, v2, *_ = gen.read_crs_bin(str(d / "CRS.ksb"))
+    assert np.array_equal(p2, ptr) and np.array_equal(c2, col) and np.array_equal(v2, val)
