@@ -24,6 +24,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# dominant kernel of the headline solve, and the committed counter summary
+# (two rocprofv3 --pmc runs, copy-calibrated) that gives its HBM traffic
+DOM_KERNEL = "ks::k_spmv_stream<float, 1, 0, ks::EpiStore<float> >"
+TRAFFIC_FILE = os.path.join("profiles", "k_spmv_stream_fp32_poisson160_traffic.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (None when the summary is absent or was made for another kernel)."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel.replace(" ", ""):
+            return float(f[2])
+    return None
 METRIC = "solve s to 1e-10 rel-res, mixed vs FP64 on GPU; SpMV HBM GB/s; adjoint grad s"
 
 
@@ -35,7 +52,8 @@ def parse():
     ap.add_argument("--grid", type=int, default=160, help="Poisson grid edge (160 = the benchmark)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GMRES comparison")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="outer steps in the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="outer steps in the CPU sample (0: the whole solve, about 11 s on 16 cores)")
     ap.add_argument("--suite", action="store_true",
                     help="also run the per-configuration SpMV, parity-solve and adjoint measurements "
                          "(one JSON line each, after the headline line)")
@@ -95,11 +113,12 @@ def main():
                    "n": n, "nnz": nnz, "outer_steps": r0["steps"], "inner_steps": r0["inner_steps"],
                    "iters": r0["iters"], "relres_true": r0["relres_true"]},
     }
-    out["roofline"] = {"bound": "hbm", "kernel": "k_spmv<float> (inner FP32 SpMV)",
+    tr = measured_traffic(DOM_KERNEL)
+    out["roofline"] = {"bound": "hbm", "kernel": DOM_KERNEL,
                        "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                       "traffic": None, "bytes_per_launch": bytes_per, "ms_per_launch": ms_per,
-                       "launches": launches}
+                       "traffic": tr, "traffic_source": TRAFFIC_FILE if tr else None,
+                       "bytes_per_launch": bytes_per, "ms_per_launch": ms_per, "launches": launches}
     out["solve_bytes_GBps"] = round(r0["bytes"] / r0["seconds"] / 1e9, 1)
 
     if not a.no_fp64:
@@ -130,17 +149,22 @@ def main():
         from oracle import oracle as O
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         kw = dict(mixed)
-        kw["outer_maxloop"] = a.cpu_steps + 1  # count+1 >= maxloop stops after cpu_steps steps
+        if a.cpu_steps:
+            kw["outer_maxloop"] = a.cpu_steps + 1  # count+1 >= maxloop stops after cpu_steps steps
         kw["omp_threads"] = threads
         bnp = P["b"]
         t = time.perf_counter()
         xc, rc, _ = O.solve(P["ptr"], P["col"], P["val"], bnp, "vpgmres", "gmres", history=False, **kw)
         tc = time.perf_counter() - t
         per_step = tc / max(rc["steps"], 1)
+        if a.cpu_steps:
+            sample = (f"oracle VPGMRES (FP32 inner) {rc['steps']} outer steps timed ({tc:.2f} s), "
+                      f"scaled to the GPU's {r0['steps']} outer steps")
+        else:
+            sample = (f"oracle VPGMRES (FP32 inner), the whole solve: {rc['steps']} outer steps, "
+                      f"relres {rc['relres_true']:.2e}")
         out["cpu_baseline"] = {"value": round(per_step * r0["steps"], 3), "unit": "s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"oracle VPGMRES (FP32 inner) {rc['steps']} outer steps timed "
-                                         f"({tc:.2f} s), scaled to the GPU's {r0['steps']} outer steps"}
+                               "kind": "port", "sample": sample}
     if a.calib_copy:
         src = torch.ones(a.calib_copy * (1 << 20) // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -190,7 +214,7 @@ def suite(ctx, a):
         if nonsym:
             rec["fp64_transpose"] = spmv_gbps(A, n, nnz, 64, trans=True)
         # the other SpMV layouts on the same matrix
-        for lay in ("wg", "vector", "merge"):
+        for lay in ("wg", "wave", "vector", "merge"):
             Al = api.Matrix(ctx, P["ptr"], P["col"], P["val"], fp32=True, layout=lay)
             rec[lay] = {"fp64": spmv_gbps(Al, n, nnz, 64), "fp32": spmv_gbps(Al, n, nnz, 32)}
             Al.close()

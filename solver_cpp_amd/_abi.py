@@ -27,6 +27,7 @@ KS_MAT_MERGE = 32
 KS_MAT_VECTOR = 64
 KS_MAT_WG_TILES = 128
 KS_MAT_BORROW = 256
+KS_MAT_WAVE_TILES = 512
 
 KS_F64 = 64
 KS_F32 = 32

@@ -172,9 +172,11 @@ class Context:
 class Matrix:
     """CSR matrix in HBM (collection's Cval/Ccol/Cptr + CTval/... for A^T)."""
 
-    # SpMV layouts: CSR-stream wavefront tiles (default), CSR-stream workgroup
-    # tiles, lanes-per-row vector kernel, merge-path
-    LAYOUTS = {"stream": 0, "wg": _abi.KS_MAT_WG_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE}
+    # SpMV layouts: CSR-stream (default: FP64 on wavefront tiles, FP32 on
+    # workgroup tiles), CSR-stream with workgroup or wavefront tiles for both
+    # value sizes, lanes-per-row vector kernel, merge-path
+    LAYOUTS = {"stream": 0, "wg": _abi.KS_MAT_WG_TILES, "wave": _abi.KS_MAT_WAVE_TILES,
+               "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE}
 
     def __init__(self, ctx, ptr, col, val, transpose=False, fp32=False, spd=False, layout="stream",
                  borrow=False):

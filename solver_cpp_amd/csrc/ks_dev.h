@@ -3,6 +3,7 @@
 // FP32 value copy, device-built transpose) and the kernel launch helpers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -108,6 +109,15 @@ struct Prof {
     KS_HIPCK(hipEventRecord(pend.back().b, s));
     cur = -1;
   }
+  // an event pair for one launch of class cls (false: cls not profiled)
+  bool pair(int cls, hipEvent_t& a, hipEvent_t& b) {
+    if (!(mask & (1u << cls))) return false;
+    Pair pr{get(), get(), cls};
+    pend.push_back(pr);
+    a = pr.a;
+    b = pr.b;
+    return true;
+  }
   // after the stream is synchronised
   void collect(double* ms) {
     for (auto& pr : pend) {
@@ -148,19 +158,26 @@ struct ks_ctx {
 
 namespace ks {
 
-// SpMV plan of one CSR matrix: CSR-stream tiles (kind 0), L lanes per row
-// (kind 1) or merge-path segments (kind 2).  For kinds 0 and 1 the tiles are
-// wavefront tiles (k_spmv_wave) unless wg_tiles (workgroup tiles,
-// k_spmv_stream); the two-input SpMV of kind 1 uses the tiles too.  For kind
-// 2, tiles / tnz hold the segment start points (row, entry) of the merge path.
-struct Plan {
-  int kind = 0;
-  bool wg_tiles = false;
-  int L = 8;
+// Tile (or merge-path segment) starts of one CSR-stream layout.
+struct TileSet {
   int64_t ntiles = 0;
   DBuf<int32_t> tiles;  // tile row starts, ntiles + 1
   DBuf<int32_t> tnz;    // tile entry starts (ptr[tiles[t]]), ntiles + 1
-  DBuf<double> carry;   // merge path: 2 * ntiles carries (FP64 or FP32)
+};
+
+// SpMV plan of one CSR matrix: CSR-stream tiles (kind 0), L lanes per row
+// (kind 1) or merge-path segments (kind 2).  Kinds 0 and 1 keep wavefront
+// tiles (k_spmv_wave) and/or workgroup tiles (k_spmv_stream); by default the
+// FP64 SpMV uses the wavefront tiles and the FP32 SpMV the workgroup tiles
+// (the faster of the two for each value size on MI355X), the k-skip
+// two-input SpMV the FP64 choice.  For kind 2, `seg` holds the segment
+// start points (row, entry) of the merge path.
+struct Plan {
+  int kind = 0;
+  bool wg64 = false, wg32 = true;  // tile set of the FP64 / FP32 SpMV
+  int L = 8;
+  TileSet wave, wg, seg;
+  DBuf<double> carry;  // merge path: 2 * segments carries (FP64 or FP32)
 };
 
 template <class T>
@@ -176,6 +193,16 @@ struct MatView {
   const int32_t* tiles = nullptr;
   const int32_t* tnz = nullptr;
   T* carry = nullptr;
+  void set_plan(const Plan& pl) {
+    kind = pl.kind;
+    L = pl.L;
+    wg_tiles = sizeof(T) == 8 ? pl.wg64 : pl.wg32;
+    const TileSet& ts = kind == 2 ? pl.seg : wg_tiles ? pl.wg : pl.wave;
+    ntiles = ts.ntiles;
+    tiles = ts.tiles.p;
+    tnz = ts.tnz.p;
+    carry = reinterpret_cast<T*>(pl.carry.p);
+  }
 };
 
 }  // namespace ks
@@ -205,8 +232,7 @@ inline ks::MatView<double> ks_mat::view<double>(bool trans) const {
   const ks::Plan& pl = trans ? tplan : plan;
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val.p; }
-  v.kind = pl.kind; v.wg_tiles = pl.wg_tiles; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
-  v.carry = reinterpret_cast<decltype(v.carry)>(pl.carry.p);
+  v.set_plan(pl);
   return v;
 }
 template <>
@@ -216,8 +242,7 @@ inline ks::MatView<float> ks_mat::view<float>(bool trans) const {
   const ks::Plan& pl = trans ? tplan : plan;
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval32.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val32.p; }
-  v.kind = pl.kind; v.wg_tiles = pl.wg_tiles; v.L = pl.L; v.ntiles = pl.ntiles; v.tiles = pl.tiles.p; v.tnz = pl.tnz.p;
-  v.carry = reinterpret_cast<decltype(v.carry)>(pl.carry.p);
+  v.set_plan(pl);
   return v;
 }
 
@@ -269,6 +294,21 @@ int grid_rows(ks_ctx* c, K kernel, int64_t nrb, size_t dyn_lds = 0) {
   return grid_rows(std::min<int64_t>(nrb, resident(c, kernel, dyn_lds)));
 }
 
+// Launches kernel<<<G, kBlock, shmem>>> on the context's stream.  When
+// kernel class `cls` is profiled, the launch carries a start/stop event pair
+// that the runtime sets to the kernel's own execution interval (the
+// dispatch's begin and end timestamps, as in a profiler's kernel trace), so
+// the per-class times exclude the gaps between kernels.
+template <class... P, class... A>
+void klaunch(ks_ctx* c, int cls, void (*kernel)(P...), int G, size_t shmem, A&&... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->prof.pair(cls, e0, e1))
+    hipExtLaunchKernelGGL(kernel, dim3(G), dim3(kBlock), (uint32_t)shmem, c->stream, e0, e1, 0u, P(args)...);
+  else
+    kernel<<<G, kBlock, shmem, c->stream>>>(P(args)...);
+  KS_HIPCK(hipGetLastError());
+}
+
 // Finisher of a reduction whose producer ran on G workgroups.
 template <int NV, class F>
 void launch_finish(ks_ctx* c, int G, const F& f, Gate g) {
@@ -279,10 +319,7 @@ void launch_finish(ks_ctx* c, int G, const F& f, Gate g) {
 template <int NV, class Op>
 void launch_map(ks_ctx* c, int64_t n, const Op& op, Gate g, int cls, double bytes) {
   const int G = grid_map(c, k_map<NV, Op>, n);
-  c->prof.begin(c->stream, cls);
-  k_map<NV, Op><<<G, kBlock, 0, c->stream>>>(n, op, g, red(c), stat(c, cls, bytes));
-  c->prof.end(c->stream);
-  KS_HIPCK(hipGetLastError());
+  klaunch(c, cls, k_map<NV, Op>, G, 0, n, op, g, red(c), stat(c, cls, bytes));
   if constexpr (NV > 0) launch_finish<NV>(c, G, op, g);
 }
 
@@ -303,11 +340,7 @@ int spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, i
   constexpr int RPB = kBlock / L;
   const int64_t nrb = (A.n + RPB - 1) / RPB;
   const int G = grid_rows(c, k_spmv<T, L, NV, Epi>, nrb);
-  c->prof.begin(c->stream, cls);
-  k_spmv<T, L, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ptr, A.col, A.val, x, epi, g, red(c),
-                                                     stat(c, cls, bytes));
-  c->prof.end(c->stream);
-  KS_HIPCK(hipGetLastError());
+  klaunch(c, cls, k_spmv<T, L, NV, Epi>, G, 0, A.n, A.ptr, A.col, A.val, x, epi, g, red(c), stat(c, cls, bytes));
   return G;
 }
 
@@ -315,18 +348,15 @@ template <class T, int NX, int NV, class Epi>
 int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
                 double bytes) {
   int G;
-  c->prof.begin(c->stream, cls);
   if (A.wg_tiles) {
     G = grid_rows(c, k_spmv_stream<T, NX, NV, Epi>, A.ntiles);
-    k_spmv_stream<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val,
-                                                               x0, x1, epi, g, red(c), stat(c, cls, bytes));
+    klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
+            epi, g, red(c), stat(c, cls, bytes));
   } else {
     G = grid_rows(c, k_spmv_wave<T, NX, NV, Epi>, (A.ntiles + kBlock / 64 - 1) / (kBlock / 64));
-    k_spmv_wave<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val,
-                                                             x0, x1, epi, g, red(c), stat(c, cls, bytes));
+    klaunch(c, cls, k_spmv_wave<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
+            epi, g, red(c), stat(c, cls, bytes));
   }
-  c->prof.end(c->stream);
-  KS_HIPCK(hipGetLastError());
   return G;
 }
 
@@ -424,28 +454,40 @@ inline void merge_segments(const int32_t* hp, int64_t n, int64_t nnz, int64_t it
   }
 }
 
-inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t nnz, bool vector, bool wg_tiles,
+inline void upload_tiles(ks_ctx* c, TileSet& ts, const std::vector<int32_t>& t, const std::vector<int32_t>& z) {
+  ts.ntiles = (int64_t)t.size() - 1;
+  ts.tiles.alloc(t.size());
+  ts.tnz.alloc(t.size());
+  KS_HIPCK(hipMemcpyAsync(ts.tiles.p, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice, c->stream));
+  KS_HIPCK(hipMemcpyAsync(ts.tnz.p, z.data(), sizeof(int32_t) * z.size(), hipMemcpyHostToDevice, c->stream));
+  KS_HIPCK(hipStreamSynchronize(c->stream));
+}
+
+inline void tile_set(ks_ctx* c, TileSet& ts, const int32_t* hp, int64_t n, int64_t rows, int64_t nnz_max) {
+  std::vector<int32_t> t = build_tiles(hp, n, rows, nnz_max), z(t.size());
+  for (size_t i = 0; i < t.size(); ++i) z[i] = hp[t[i]];
+  upload_tiles(c, ts, t, z);
+}
+
+// tiles: 0 default (FP64 wavefront, FP32 workgroup tiles), 1 workgroup tiles
+// for both, 2 wavefront tiles for both.
+inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t nnz, bool vector, int tiles,
                       bool merge = false) {
   pl.L = 2;
   const double mean = n > 0 ? (double)nnz / (double)n : 1.0;
   while (pl.L < 64 && pl.L < mean) pl.L *= 2;
   pl.kind = merge ? 2 : vector ? 1 : 0;
-  pl.wg_tiles = wg_tiles && !merge;
-  std::vector<int32_t> t, z;
   if (merge) {
+    std::vector<int32_t> t, z;
     merge_segments(hp, n, nnz, kMergeItems, t, z);
     pl.carry.alloc(2 * (t.size() - 1));
-  } else {
-    t = wg_tiles ? build_tiles(hp, n, kTileRows, kTileNnz) : build_tiles(hp, n, kWaveRows, kWaveNnz);
-    z.resize(t.size());
-    for (size_t i = 0; i < t.size(); ++i) z[i] = hp[t[i]];
+    upload_tiles(c, pl.seg, t, z);
+    return;
   }
-  pl.ntiles = (int64_t)t.size() - 1;
-  pl.tiles.alloc(t.size());
-  pl.tnz.alloc(t.size());
-  KS_HIPCK(hipMemcpyAsync(pl.tiles.p, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice, c->stream));
-  KS_HIPCK(hipMemcpyAsync(pl.tnz.p, z.data(), sizeof(int32_t) * z.size(), hipMemcpyHostToDevice, c->stream));
-  KS_HIPCK(hipStreamSynchronize(c->stream));
+  pl.wg64 = tiles == 1;
+  pl.wg32 = tiles != 2;
+  if (!pl.wg64 || !pl.wg32) tile_set(c, pl.wave, hp, n, kWaveRows, kWaveNnz);
+  if (pl.wg64 || pl.wg32) tile_set(c, pl.wg, hp, n, kTileRows, kTileNnz);
 }
 
 // Lanes per row: smallest power of two >= mean row length, in [2, 64].

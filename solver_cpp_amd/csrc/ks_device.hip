@@ -105,6 +105,13 @@ struct OpResid2 {  // (r,r) of an existing residual
 };
 
 // ---------------------------------------------------------- transpose ---
+// CSR-stream tile sets requested by the matrix flags (make_plan's `tiles`).
+int tile_choice(uint32_t flags) {
+  if (flags & KS_MAT_WG_TILES) return 1;
+  if (flags & KS_MAT_WAVE_TILES) return 2;
+  return 0;
+}
+
 void build_transpose(ks_mat* m) {
   ks_ctx* c = m->ctx;
   const int64_t n = m->n, nnz = m->nnz;
@@ -155,7 +162,7 @@ void build_transpose(ks_mat* m) {
     k_gather<double, double><<<grid_map(nnz), kBlock, 0, s>>>(nnz, m->tperm.p, m->val.p, m->tval.p);
     KS_HIPCK(hipGetLastError());
   }
-  make_plan(c, m->tplan, htp.data(), n, nnz, (m->flags & KS_MAT_VECTOR) != 0, (m->flags & KS_MAT_WG_TILES) != 0,
+  make_plan(c, m->tplan, htp.data(), n, nnz, (m->flags & KS_MAT_VECTOR) != 0, tile_choice(m->flags),
             (m->flags & KS_MAT_MERGE) != 0);
   m->has_t = true;
 }
@@ -514,7 +521,7 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   for (int64_t j = 0; j < nnz; ++j)
     KS_CHECK(hc[j] >= 0 && hc[j] < n, KS_ERR_BAD_INPUT, "column index out of range at entry " + std::to_string(j));
   m->max_row = mx;
-  make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0, (flags & KS_MAT_WG_TILES) != 0,
+  make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0, tile_choice(flags),
               (flags & KS_MAT_MERGE) != 0);
   if (flags & KS_MAT_TRANSPOSE) build_transpose(m.get());
   if (flags & KS_MAT_FP32_COPY) ensure_fp32(m.get());

@@ -636,16 +636,13 @@ template <class T, int GV>
 void proj_launch(ks_ctx* c, int64_t n, const T* Vi, int64_t ldv, int m, const T* w, double* hi, int ldh, Gate g) {
   const double bytes = (double)sizeof(T) * n * (m + 1);
   int G;
-  c->prof.begin(c->stream, KS_K_CGS);
   if (vec_ok(Vi, ldv, w)) {
     G = grid_map(c, k_proj<T, GV, true>, (n + Vec16<T>::N - 1) / Vec16<T>::N);
-    k_proj<T, GV, true><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
+    klaunch(c, KS_K_CGS, k_proj<T, GV, true>, G, 0, n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
   } else {
     G = grid_map(c, k_proj<T, GV, false>, n);
-    k_proj<T, GV, false><<<G, kBlock, 0, c->stream>>>(n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
+    klaunch(c, KS_K_CGS, k_proj<T, GV, false>, G, 0, n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
   }
-  c->prof.end(c->stream);
-  KS_HIPCK(hipGetLastError());
   launch_finish<GV>(c, G, FinProj{hi, ldh, m}, g);
 }
 
@@ -675,18 +672,15 @@ void launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, T*
   const double bytes = (double)sizeof(T) * n * (nv + 2);
   const size_t lds = sizeof(double) * std::max(nv, 1);
   int G;
-  c->prof.begin(c->stream, KS_K_CGS);
   if (vec_ok(V, ldv, (const T*)w)) {
     G = grid_map(c, k_cgs_update<T, true>, (n + Vec16<T>::N - 1) / Vec16<T>::N, lds);
-    k_cgs_update<T, true><<<G, kBlock, lds, c->stream>>>(n, V, ldv, nv, w, hcol, ldh, g, red(c),
-                                                         stat(c, KS_K_CGS, bytes));
+    klaunch(c, KS_K_CGS, k_cgs_update<T, true>, G, lds, n, V, ldv, nv, w, hcol, ldh, g, red(c),
+            stat(c, KS_K_CGS, bytes));
   } else {
     G = grid_map(c, k_cgs_update<T, false>, n, lds);
-    k_cgs_update<T, false><<<G, kBlock, lds, c->stream>>>(n, V, ldv, nv, w, hcol, ldh, g, red(c),
-                                                          stat(c, KS_K_CGS, bytes));
+    klaunch(c, KS_K_CGS, k_cgs_update<T, false>, G, lds, n, V, ldv, nv, w, hcol, ldh, g, red(c),
+            stat(c, KS_K_CGS, bytes));
   }
-  c->prof.end(c->stream);
-  KS_HIPCK(hipGetLastError());
   launch_finish<1>(c, G, FinWrap<Fin>{f}, g);
 }
 
@@ -777,17 +771,13 @@ struct GmresLevel : Level {
     KS_HIPCK(hipGetLastError());
     const int last = (cfg.compat & KS_COMPAT_LAST_TERM) ? 1 : 0;
     const double bytes = (double)sizeof(T) * n * 2 + (inner ? 8.0 : (double)sizeof(T)) * n * (R - 1);
-    c->prof.begin(c->stream, KS_K_CGS);
+    const size_t lds = sizeof(double) * R;
     if (inner)
-      k_assemble<T, double><<<grid_map(c, k_assemble<T, double>, n, sizeof(double) * R), kBlock,
-                               sizeof(double) * R, c->stream>>>(
-          n, Z.p, ldv, x, h, last, gdone(), stat(c, KS_K_CGS, bytes));
+      klaunch(c, KS_K_CGS, k_assemble<T, double>, grid_map(c, k_assemble<T, double>, n, lds), lds, n, Z.p, ldv, x,
+              h, last, gdone(), stat(c, KS_K_CGS, bytes));
     else
-      k_assemble<T, T><<<grid_map(c, k_assemble<T, T>, n, sizeof(double) * R), kBlock, sizeof(double) * R,
-                          c->stream>>>(
-          n, V.p, ldv, x, h, last, gdone(), stat(c, KS_K_CGS, bytes));
-    c->prof.end(c->stream);
-    KS_HIPCK(hipGetLastError());
+      klaunch(c, KS_K_CGS, k_assemble<T, T>, grid_map(c, k_assemble<T, T>, n, lds), lds, n, V.p, ldv, x, h, last,
+              gdone(), stat(c, KS_K_CGS, bytes));
     k_mark_done<<<1, 1, 0, c->stream>>>(h, gdone());
     KS_HIPCK(hipGetLastError());
   }
@@ -952,13 +942,10 @@ struct GcrLevel : Level {
     launch_spmv<T, 0>(c, A, src, EpiStore<T>{Av.p}, g);
     launch_proj<T>(c, n, Q.p, ldv, k + 1, Av.p, hctl.y, 1, g);
     const double bytes = (double)sizeof(T) * n * (2.0 * (k + 1) + 4.0);
-    c->prof.begin(c->stream, KS_K_CGS);
-    k_gcr_update<T><<<grid_map(c, k_gcr_update<T>, n, sizeof(double) * (k + 1)), kBlock, sizeof(double) * (k + 1),
-                      c->stream>>>(
-        n, P.p, Q.p, ldv, k + 1, src, Av.p, hctl.y, hctl.e, P.p + (int64_t)(k + 1) * ldv,
-        Q.p + (int64_t)(k + 1) * ldv, g, stat(c, KS_K_CGS, bytes));
-    c->prof.end(c->stream);
-    KS_HIPCK(hipGetLastError());
+    const size_t lds = sizeof(double) * (k + 1);
+    klaunch(c, KS_K_CGS, k_gcr_update<T>, grid_map(c, k_gcr_update<T>, n, lds), lds, n, P.p, Q.p, ldv, k + 1, src,
+            Av.p, hctl.y, hctl.e, P.p + (int64_t)(k + 1) * ldv, Q.p + (int64_t)(k + 1) * ldv, g,
+            stat(c, KS_K_CGS, bytes));
   }
   void enqueue_finish() override {
     if (sizeof(T) != 8) {

@@ -2,6 +2,16 @@
 
   python -m solver_cpp_amd.rocpd_summary trace <results.db> [--top N]
   python -m solver_cpp_amd.rocpd_summary pmc <results.db> [--kernel SUBSTR]
+  python -m solver_cpp_amd.rocpd_summary traffic <fetch.db> --write-db <write.db>
+         --kernel NAME --copy-bytes B
+
+`traffic` turns one FETCH_SIZE run and one WRITE_SIZE run into HBM bytes per
+launch of one kernel: each counter is scaled by what it reads for the plain
+device copy of B bytes in the same run (the largest copyBuffer dispatch;
+B/2 read and B/2 written), and the kernel's median over its working
+dispatches (FETCH_SIZE above 1 MiB, WRITE_SIZE above 64 KiB: gated
+launches that returned at once are left out) is reported, with a final machine-readable line
+`traffic_bytes_per_launch <kernel> <bytes>`.
 
 `trace` lists kernels by total time (calls, total and mean duration in us;
 dispatches faster than --active-us are also reported separately: those are
@@ -53,15 +63,47 @@ def pmc(db, match=None):
     return "\n".join(out)
 
 
+def _counter(db, counter):
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    return [(short(k), v) for k, c, v in rows if c == counter]
+
+
+def traffic(fetch_db, write_db, kernel, copy_bytes):
+    fr, wr = _counter(fetch_db, "FETCH_SIZE"), _counter(write_db, "WRITE_SIZE")
+    cf = max(v for k, v in fr if "copyBuffer" in k)
+    cw = max(v for k, v in wr if "copyBuffer" in k)
+    f_factor = (copy_bytes / 2) / (cf * 1024.0)
+    w_factor = (copy_bytes / 2) / (cw * 1024.0)
+    kf = [v for k, v in fr if k == kernel and v > 1024]
+    kw = [v for k, v in wr if k == kernel and v > 64]
+    mf, mw = statistics.median(kf), statistics.median(kw)
+    t = mf * 1024.0 * f_factor + mw * 1024.0 * w_factor
+    out = [f"# calibration: copy of {copy_bytes} bytes: FETCH_SIZE {cf:.0f} KiB (factor {f_factor:.3f}), "
+           f"WRITE_SIZE {cw:.0f} KiB (factor {w_factor:.3f})",
+           f"# {kernel}: {len(kf)} working dispatches, median FETCH_SIZE {mf:.1f} KiB, WRITE_SIZE {mw:.1f} KiB",
+           f"# HBM bytes per launch = {mf:.1f} KiB x {f_factor:.3f} + {mw:.1f} KiB x {w_factor:.3f} "
+           f"= {t / 1e6:.1f} MB",
+           f"traffic_bytes_per_launch {kernel.replace(' ', '')} {t:.0f}"]
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "pmc"])
+    ap.add_argument("mode", choices=["trace", "pmc", "traffic"])
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=20)
     ap.add_argument("--active-us", type=float, default=10.0)
     ap.add_argument("--kernel", default=None)
+    ap.add_argument("--write-db", default=None)
+    ap.add_argument("--copy-bytes", type=float, default=0)
     a = ap.parse_args()
-    print(trace(a.db, a.top, a.active_us, a.kernel) if a.mode == "trace" else pmc(a.db, a.kernel))
+    if a.mode == "trace":
+        print(trace(a.db, a.top, a.active_us, a.kernel))
+    elif a.mode == "pmc":
+        print(pmc(a.db, a.kernel))
+    else:
+        print(traffic(a.db, a.write_db, a.kernel, a.copy_bytes))
 
 
 if __name__ == "__main__":

@@ -21,7 +21,7 @@ def _mat(ctx, ptr, col, val, **kw):
     return api.Matrix(ctx, ptr, col, val, **kw)
 
 
-LAYOUTS = ["stream", "wg", "vector", "merge"]
+LAYOUTS = ["stream", "wg", "wave", "vector", "merge"]
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
@@ -36,7 +36,7 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     n = len(ptr) - 1
     x = np.random.default_rng(7).standard_normal(n)
     A = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
-    assert A.info()["spmv_kind"] == {"stream": 0, "wg": 0, "vector": 1, "merge": 2}[layout]
+    assert A.info()["spmv_kind"] == {"stream": 0, "wg": 0, "wave": 0, "vector": 1, "merge": 2}[layout]
     m = np.diff(ptr).max()
     y = A.spmv(x)
     yo = oracle.spmv(ptr, col, val, x)
@@ -61,7 +61,7 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
 
 
-@pytest.mark.parametrize("layout", ["stream", "wg"])
+@pytest.mark.parametrize("layout", ["stream", "wg", "wave"])
 @pytest.mark.parametrize("case", ["poisson2d", "convdiff3d", "aniso"])
 def test_spmv_bitexact_short_rows(ctx, oracle, case, layout):
     # rows of <= 8 entries are summed sequentially in CSR order from products
@@ -203,7 +203,7 @@ def test_cgs_step(ctx, oracle, k, n):
     assert np.abs(V @ wg).max() <= 1e-12 * np.linalg.norm(w)
 
 
-@pytest.mark.parametrize("layout", ["stream", "wg", "merge"])
+@pytest.mark.parametrize("layout", ["stream", "wg", "wave", "merge"])
 @pytest.mark.parametrize("k", [1, 2, 4])
 def test_kskip_basis_and_gram(ctx, oracle, k, layout):
     ptr, col, val, _, _ = P.convdiff3d(10, pe=0.0)

@@ -203,10 +203,14 @@ def test_adjoint_small(ctx, oracle):
     assert np.allclose(gv, -lam[rows] * x[col], rtol=0, atol=1e-9 * np.abs(lam).max() * np.abs(x).max())
 
 
-@pytest.mark.parametrize("layout", ["wg", "vector", "merge"])
+@pytest.mark.parametrize("layout", ["wg", "wave", "vector", "merge"])
 def test_solvers_on_every_spmv_layout(ctx, oracle, layout):
     # the fused SpMV epilogues (residual + norm, dot) and the two-input
-    # k-skip basis on each SpMV layout give the default layout's results
+    # k-skip basis on each SpMV layout give the default layout's results.
+    # FP64 solvers: iterations +-1.  The mixed case on Zipf rows (up to 32
+    # entries) sums FP32 rows in a layout-dependent order, which perturbs the
+    # FP32 inner preconditioner at ~1e-7 and the outer count by a few steps:
+    # +-3 there, x still within 1e-7.
     from solver_cpp_amd import api
     cases = [("cg", "NO", P.poisson2d(30)[:4], {}),
              ("gmres", "NO", P.convdiff3d(11)[:4], dict(outer_restart=11)),
@@ -221,10 +225,11 @@ def test_solvers_on_every_spmv_layout(ctx, oracle, layout):
         x0, r0, _ = api.solve(ctx, A0, b, outer, inner, **kw)
         x1, r1, _ = api.solve(ctx, A1, b, outer, inner, **kw)
         assert r0["status"] == 0 and r1["status"] == 0, (outer, r0, r1)
-        assert abs(r0["iters"] - r1["iters"]) <= 1, (outer, r0["iters"], r1["iters"])
+        tol = 3 if kw.get("inner_precision") == 32 else 1
+        assert abs(r0["iters"] - r1["iters"]) <= tol, (outer, r0["iters"], r1["iters"])
         assert np.linalg.norm(x1 - x0) <= 1e-7 * np.linalg.norm(x0), outer
         xo, ro, _ = oracle.solve(ptr, col, val, b, outer, inner, **kw)
-        assert abs(r1["iters"] - ro["iters"]) <= 1, (outer, r1["iters"], ro["iters"])
+        assert abs(r1["iters"] - ro["iters"]) <= tol, (outer, r1["iters"], ro["iters"])
 
 
 def test_cli_solver_end_to_end(tmp_path, oracle):
