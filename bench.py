@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # dominant kernel of the headline solve, and the committed counter summary
 # (two rocprofv3 --pmc runs, copy-calibrated) that gives its HBM traffic
-DOM_KERNEL = "ks::k_spmv_stream<float, 1, 0, ks::EpiStore<float> >"
+DOM_KERNEL = "ks::k_spmv_stream<float, 1, 0, ks::EpiDiv<float> >"
 TRAFFIC_FILE = os.path.join("profiles", "k_spmv_stream_fp32_poisson160_traffic.txt")
 
 

@@ -506,6 +506,17 @@ struct EpiStore {
   __device__ void row(int64_t i, T s, T, double*) { y[i] = s; }
   __device__ void fin(const double*) const {}
 };
+// y = (A x) * (1 / *d) (d: device FP64 scalar; the reciprocal is formed
+// once per thread, in T)
+template <class T>
+struct EpiDiv {
+  T* y;
+  const double* d;
+  T inv;
+  __device__ void prep() { inv = (T)(1.0 / *d); }
+  __device__ void row(int64_t i, T s, T, double*) { y[i] = s * inv; }
+  __device__ void fin(const double*) const {}
+};
 template <class T>
 struct EpiStore2 {
   T *y0, *y1;

@@ -706,7 +706,7 @@ ks_status ks_cgs(ks_ctx* c, int64_t n, int32_t k, const double* V, double* w, do
   Out<double> wo(c, w, n, flags, true);
   DBuf<double> hd(k + 2);
   launch_proj<double>(c, n, Vi.p, n, k + 1, wo.p, hd.p, 1, Gate{});
-  launch_cgs_update<double>(c, n, Vi.p, n, k + 1, wo.p, hd.p, 1, FinStore{hd.p + k + 1, 1}, Gate{});
+  launch_cgs_update<double>(c, n, Vi.p, n, k + 1, wo.p, wo.p, hd.p, 1, FinStore{hd.p + k + 1, 1}, Gate{});
   wo.copy_back(c);
   KS_HIPCK(hipMemcpyAsync(h, hd.p, sizeof(double) * (k + 2), hipMemcpyDeviceToHost, c->stream));
   KS_HIPCK(hipStreamSynchronize(c->stream));

@@ -48,6 +48,7 @@ struct Ctl {
   int* host_stop;  // host-mapped flag mirror of `stop` (nullable)
   int R, ld;       // GMRES restart and H row stride
   double *H, *e, *cs, *sn, *y;
+  double* vn;      // GMRES: norms of the unnormalised basis vectors (R)
   int kskip, fix, compat, pad;
   double *delta, *eta, *zeta, *rho, *phi;
 };
@@ -803,24 +804,29 @@ __global__ __launch_bounds__(kBlock) void k_proj(int64_t n, const T* __restrict_
   }
   block_store<G>(acc, ra);
 }
-// Finisher of k_proj: h column entries.
+// Finisher of k_proj: h column entries.  With vn set, column v of the basis
+// is stored unnormalised with norm vn[v], and h_v = (w, V_v) / vn[v].
 struct FinProj {
   double* hout;
   int ldh, nv;
+  const double* vn;
   __device__ void fin(const double* s) const {
-    for (int v = 0; v < nv; ++v) hout[(int64_t)v * ldh] = s[v];
+    for (int v = 0; v < nv; ++v) hout[(int64_t)v * ldh] = vn ? s[v] / vn[v] : s[v];
   }
 };
 
-// w -= sum_{i<nv} h_i V_i (classical GS update), then ||w||^2 -> fin.
+// out = w - sum_{i<nv} h_i V_i (classical GS update; out may be w), then
+// ||out||^2 -> fin.  With vn set, column i of V is stored unnormalised with
+// norm vn[i] and enters as h_i / vn[i].
 template <class T, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __restrict__ V, int64_t ldv,
-                                                       int nv, T* __restrict__ w, const double* hcol,
-                                                       int ldh, Gate g, RedArgs ra, StatArgs st) {
+                                                       int nv, const T* w, T* out, const double* hcol,
+                                                       int ldh, const double* vn, Gate g, RedArgs ra,
+                                                       StatArgs st) {
   if (g.off()) return;
   st.count();
   extern __shared__ double sh[];
-  for (int v = threadIdx.x; v < nv; v += kBlock) sh[v] = hcol[(int64_t)v * ldh];
+  for (int v = threadIdx.x; v < nv; v += kBlock) sh[v] = vn ? hcol[(int64_t)v * ldh] / vn[v] : hcol[(int64_t)v * ldh];
   __syncthreads();
   double acc[1] = {0.0};
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -830,7 +836,8 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
     using VT = typename Vec16<T>::type;
     constexpr int N = Vec16<T>::N;
     const int64_t nvec = n / N;
-    VT* wv = reinterpret_cast<VT*>(w);
+    const VT* wv = reinterpret_cast<const VT*>(w);
+    VT* ov = reinterpret_cast<VT*>(out);
     for (int64_t i = gid; i < nvec; i += stride) {
       const VT a = wv[i];
       double t[N];
@@ -849,7 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
         o[e] = (T)t[e];
         acc[0] += (double)o[e] * (double)o[e];
       }
-      wv[i] = o;
+      ov[i] = o;
     }
     i0 = nvec * N + gid;
   }
@@ -858,7 +865,7 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
 #pragma unroll 8
     for (int v = 0; v < nv; ++v) t -= sh[v] * (double)V[(int64_t)v * ldv + i];
     const T tw = (T)t;
-    w[i] = tw;
+    out[i] = tw;
     acc[0] += (double)tw * (double)tw;
   }
   block_store<1>(acc, ra);
@@ -867,10 +874,11 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
 // x += sum_{i<size} y_i B_i   (or only the last term with KS_COMPAT_LAST_TERM
 // when the cycle stopped inside); y and size come from the Ctl (k_hye).
 // Tx: type of x, Tb: type of the basis B.
+// With c->vn set (and use_vn), B holds the unnormalised basis: y_i / vn[i].
 template <class Tx, class Tb>
 __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __restrict__ B, int64_t ldb,
                                                      Tx* __restrict__ x, const Ctl* c, int last_term,
-                                                     Gate g, StatArgs st) {
+                                                     int use_vn, Gate g, StatArgs st) {
   if (g.off()) return;
   st.count();
   extern __shared__ double sy[];
@@ -881,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __rest
   }
   __syncthreads();
   const int sz = s_sz;
-  for (int v = threadIdx.x; v < sz; v += kBlock) sy[v] = c->y[v];
+  for (int v = threadIdx.x; v < sz; v += kBlock) sy[v] = use_vn ? c->y[v] / c->vn[v] : c->y[v];
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   if (s_lt) {

@@ -263,6 +263,7 @@ struct FinGmCycle {
     for (int i = 0; i < c->R; ++i) c->e[i] = 0.0;
     c->e[0] = t;
     c->vscale = t;
+    c->vn[0] = t;
     gm_check(c, 0, t);
   }
 };
@@ -278,6 +279,7 @@ struct FinGmNorm {
     const double h = sqrt(s[0]);
     c->H[(k + 1) * c->ld + k] = h;
     c->vscale = h;
+    c->vn[k + 1] = h;
     givens_step(c, k);
   }
 };
@@ -408,7 +410,7 @@ struct Level {
     hctl.fix = cfg.fix;
     hctl.compat = cfg.compat;
     const int64_t nH = R > 0 ? (int64_t)(R + 1) * R : 0;
-    const int64_t nsmall = R > 0 ? 4 * (int64_t)R : 0;
+    const int64_t nsmall = R > 0 ? 5 * (int64_t)R : 0;
     const int64_t nk = 5 * (int64_t)(2 * k + 3);
     arr.alloc((size_t)(nH + nsmall + nk + 8));
     KS_HIPCK(hipMemsetAsync(arr.p, 0, arr.n * sizeof(double), c->stream));
@@ -419,6 +421,7 @@ struct Level {
       hctl.cs = q; q += R;
       hctl.sn = q; q += R;
       hctl.y = q; q += R;
+      hctl.vn = q; q += R;
     }
     hctl.delta = q; q += 2 * k + 3;
     hctl.eta = q; q += 2 * k + 3;
@@ -633,7 +636,8 @@ bool vec_ok(const T* V, int64_t ldv, const T* w) {
 }
 
 template <class T, int GV>
-void proj_launch(ks_ctx* c, int64_t n, const T* Vi, int64_t ldv, int m, const T* w, double* hi, int ldh, Gate g) {
+void proj_launch(ks_ctx* c, int64_t n, const T* Vi, int64_t ldv, int m, const T* w, double* hi, int ldh, Gate g,
+                 const double* vn) {
   const double bytes = (double)sizeof(T) * n * (m + 1);
   int G;
   if (vec_ok(Vi, ldv, w)) {
@@ -643,20 +647,22 @@ void proj_launch(ks_ctx* c, int64_t n, const T* Vi, int64_t ldv, int m, const T*
     G = grid_map(c, k_proj<T, GV, false>, n);
     klaunch(c, KS_K_CGS, k_proj<T, GV, false>, G, 0, n, Vi, ldv, m, w, g, red(c), stat(c, KS_K_CGS, bytes));
   }
-  launch_finish<GV>(c, G, FinProj{hi, ldh, m}, g);
+  launch_finish<GV>(c, G, FinProj{hi, ldh, m, vn}, g);
 }
 
+// vn (device, nullable): norms of unnormalised basis columns.
 template <class T>
 void launch_proj(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T* w, double* hcol, int ldh,
-                 Gate g) {
+                 Gate g, const double* vn = nullptr) {
   for (int i0 = 0; i0 < nv; i0 += 32) {
     const int m = std::min(32, nv - i0);
     const T* Vi = V + (int64_t)i0 * ldv;
     double* hi = hcol + (int64_t)i0 * ldh;
-    if (m <= 4) proj_launch<T, 4>(c, n, Vi, ldv, m, w, hi, ldh, g);
-    else if (m <= 8) proj_launch<T, 8>(c, n, Vi, ldv, m, w, hi, ldh, g);
-    else if (m <= 16) proj_launch<T, 16>(c, n, Vi, ldv, m, w, hi, ldh, g);
-    else proj_launch<T, 32>(c, n, Vi, ldv, m, w, hi, ldh, g);
+    const double* vi = vn ? vn + i0 : nullptr;
+    if (m <= 4) proj_launch<T, 4>(c, n, Vi, ldv, m, w, hi, ldh, g, vi);
+    else if (m <= 8) proj_launch<T, 8>(c, n, Vi, ldv, m, w, hi, ldh, g, vi);
+    else if (m <= 16) proj_launch<T, 16>(c, n, Vi, ldv, m, w, hi, ldh, g, vi);
+    else proj_launch<T, 32>(c, n, Vi, ldv, m, w, hi, ldh, g, vi);
   }
 }
 
@@ -666,19 +672,20 @@ struct FinWrap {
   __device__ void fin(const double* s) const { f(s); }
 };
 
+// out = w - V h (out may be w); vn as in launch_proj.
 template <class T, class Fin>
-void launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, T* w, const double* hcol, int ldh,
-                       Fin f, Gate g) {
+void launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T* w, T* out,
+                       const double* hcol, int ldh, Fin f, Gate g, const double* vn = nullptr) {
   const double bytes = (double)sizeof(T) * n * (nv + 2);
   const size_t lds = sizeof(double) * std::max(nv, 1);
   int G;
-  if (vec_ok(V, ldv, (const T*)w)) {
+  if (vec_ok(V, ldv, w) && vec_ok(V, ldv, (const T*)out)) {
     G = grid_map(c, k_cgs_update<T, true>, (n + Vec16<T>::N - 1) / Vec16<T>::N, lds);
-    klaunch(c, KS_K_CGS, k_cgs_update<T, true>, G, lds, n, V, ldv, nv, w, hcol, ldh, g, red(c),
+    klaunch(c, KS_K_CGS, k_cgs_update<T, true>, G, lds, n, V, ldv, nv, w, out, hcol, ldh, vn, g, red(c),
             stat(c, KS_K_CGS, bytes));
   } else {
     G = grid_map(c, k_cgs_update<T, false>, n, lds);
-    klaunch(c, KS_K_CGS, k_cgs_update<T, false>, G, lds, n, V, ldv, nv, w, hcol, ldh, g, red(c),
+    klaunch(c, KS_K_CGS, k_cgs_update<T, false>, G, lds, n, V, ldv, nv, w, out, hcol, ldh, vn, g, red(c),
             stat(c, KS_K_CGS, bytes));
   }
   launch_finish<1>(c, G, FinWrap<Fin>{f}, g);
@@ -705,7 +712,8 @@ struct GmresLevel : Level {
     KS_CHECK(R >= 2, KS_ERR_BAD_INPUT, "GMRES restart (-R/-r) must be >= 2");
     ldv = pad64(n);
     V.alloc((size_t)ldv * R);
-    w.alloc(n); r.alloc(n);
+    w.alloc(n);
+    if (in) r.alloc(n);  // plain GMRES forms r in V_0
     if (sizeof(T) != 8) xown.alloc(n);
     setup_ctl(R, 0);
     inner = std::move(in);
@@ -733,36 +741,45 @@ struct GmresLevel : Level {
     x = sizeof(T) == 8 ? (T*)x_out : xown.p;
     if (x0given) launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
   }
+  // Plain GMRES keeps its basis unnormalised: V_k holds w_k with norm
+  // vn[k], the SpMV of step k divides its rows by vn[k], and the projection,
+  // the update and the assembly take the norms into their coefficients, so no
+  // separate v = w / ||w|| pass runs.  VPGMRES normalises V explicitly: its
+  // inner solve takes v_k as a right-hand side.
   void cycle_start(int64_t cyc) {
+    T* r0 = inner ? r.p : V.p;
     if (cyc == 0 && !x0given) {
       OpGmFirstFin<T> op;
-      op.src = b; op.r = r.p; op.x = x; op.f = FinGmCycle{ctl(), 1, 1};
+      op.src = b; op.r = r0; op.x = x; op.f = FinGmCycle{ctl(), 1, 1};
       launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (8.0 + 2.0 * sizeof(T)) * n);
     } else {
       Gate g = cyc == 0 ? gouter() : gstep();
-      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinGmCycle>{b, r.p, FinGmCycle{ctl(), cyc == 0 ? 1 : 0, 0}}, g, 8.0 * n);
+      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinGmCycle>{b, r0, FinGmCycle{ctl(), cyc == 0 ? 1 : 0, 0}}, g, 8.0 * n);
     }
-    launch_map<0>(c, n, OpScale<T>{r.p, V.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
+    if (inner) launch_map<0>(c, n, OpScale<T>{r.p, V.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
   }
   void enqueue_unit(int64_t u) override {
     const int64_t cyc = u / (R - 1);
     const int k = (int)(u % (R - 1));
     if (k == 0) cycle_start(cyc);
     T* vk = V.p + (int64_t)k * ldv;
+    T* vk1 = V.p + (int64_t)(k + 1) * ldv;
+    Ctl* h = ctl();
     if (inner) {
       double* zk = Z.p + (int64_t)k * ldv;
       inner->enqueue_init((const double*)vk, zk);
       for (int64_t iu = 0; iu < inner->max_units(); ++iu) inner->enqueue_unit(iu);
       inner->enqueue_finish();
       launch_spmv<T, 0>(c, A, (const T*)zk, EpiStore<T>{w.p}, gstep());
+      launch_proj<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, gstep());
+      launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
+      launch_map<0>(c, n, OpScale<T>{w.p, vk1, h, 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
     } else {
-      launch_spmv<T, 0>(c, A, vk, EpiStore<T>{w.p}, gstep());
+      launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w.p, hctl.vn + k}, gstep());
+      launch_proj<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, gstep(), hctl.vn);
+      launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, vk1, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep(),
+                           hctl.vn);
     }
-    Ctl* h = ctl();
-    launch_proj<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, gstep());
-    launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
-    launch_map<0>(c, n, OpScale<T>{w.p, V.p + (int64_t)(k + 1) * ldv, h, 0.0}, gstep(), KS_K_AXPY,
-                  2.0 * sizeof(T) * n);
     if (k == R - 2) cycle_end();
   }
   void cycle_end() {
@@ -774,10 +791,10 @@ struct GmresLevel : Level {
     const size_t lds = sizeof(double) * R;
     if (inner)
       klaunch(c, KS_K_CGS, k_assemble<T, double>, grid_map(c, k_assemble<T, double>, n, lds), lds, n, Z.p, ldv, x,
-              h, last, gdone(), stat(c, KS_K_CGS, bytes));
+              h, last, 0, gdone(), stat(c, KS_K_CGS, bytes));
     else
       klaunch(c, KS_K_CGS, k_assemble<T, T>, grid_map(c, k_assemble<T, T>, n, lds), lds, n, V.p, ldv, x, h, last,
-              gdone(), stat(c, KS_K_CGS, bytes));
+              1, gdone(), stat(c, KS_K_CGS, bytes));
     k_mark_done<<<1, 1, 0, c->stream>>>(h, gdone());
     KS_HIPCK(hipGetLastError());
   }
