@@ -255,11 +255,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
 // CSR-stream SpMV.  The rows are cut (on the host, once per matrix) into
 // tiles of at most kTileRows rows and kTileNnz entries; a row longer than
 // kTileNnz is a tile of its own.  tiles[t] / tnz[t] are the first row and
-// first entry of tile t.  Per tile the workgroup (1) multiplies the tile's
-// values by the gathered x and writes the products to LDS, then (2) sums
-// each row's products from LDS with `tpr` threads per row (tpr = 1 for rows
-// of up to 8 entries: a sequential sum in CSR order, the reference's
-// blas::MtxVec_mult order, blas.cpp:56-62).  The loop is software-pipelined:
+// first entry of tile t.  Per tile the workgroup (1) stages the tile's
+// values and column indices in LDS (read from HBM coalesced, consecutive
+// entries on consecutive lanes), then (2) sums each row from LDS with `tpr`
+// threads per row (tpr = 1 for rows of up to 8 entries: a sequential sum in
+// CSR order, the reference's blas::MtxVec_mult order, blas.cpp:56-62),
+// gathering x as it goes.  With one row per lane, lane l gathers x for the
+// j-th entry of row r0 + l, so on stencil-like matrices one gather
+// instruction reads a contiguous run of x instead of the ~7 separate bands
+// that consecutive entries touch.  The loop is software-pipelined:
 // while tile t is summed, the bounds, row pointers, values and column
 // indices of the workgroup's next tile are already in flight (eight
 // independent loads per thread).  NX = 2 forms A x0 and A x1 from one pass
@@ -270,6 +274,14 @@ constexpr int kTileUnroll = kTileNnz / kBlock;
 
 template <class T>
 __device__ inline T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+
+// a + v * x with the product rounded to T before the add (no FMA): the
+// reference's `out += val * in` order (blas.cpp:56-62).
+template <class T>
+__device__ inline T mul_add_rn(T a, T v, T x) {
+#pragma clang fp contract(off)
+  return a + v * x;
+}
 
 template <class T>
 struct TileData {
@@ -298,9 +310,12 @@ struct TileData {
 // occupancy each reaches without spilling.
 template <class T, int NX, int NV>
 constexpr int kStreamWaves = sizeof(T) == 4 ? (NV == 0 ? 8 : 7) : (NX == 2 ? 4 : 5);
+// The workgroup kernel's FP32 variants need one more step (72 / 80 VGPRs).
+template <class T, int NX, int NV>
+constexpr int kWgWaves = sizeof(T) == 4 ? (NV == 0 ? 7 : 6) : kStreamWaves<T, NX, NV>;
 
 template <class T, int NX, int NV, class Epi>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamWaves<T, NX, NV>))) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves<T, NX, NV>))) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
                                                         const int32_t* __restrict__ tnz,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ col,
@@ -310,7 +325,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
   if (g.off()) return;
   st.count();
   epi.prep();
-  __shared__ T prod[NX][kTileNnz];
+  __shared__ T sv[kTileNnz];
+  __shared__ int32_t sc[kTileNnz];
   __shared__ int32_t sptr[kTileRows + 1];
   __shared__ double slong[NX][kBlock / 64];
   double acc[NV > 0 ? NV : 1];
@@ -375,8 +391,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
       for (int q = 0; q < kTileUnroll; ++q) {
         const int k = tid + q * kBlock;
         if (k < cnt) {
-          prod[0][k] = d.v[q] * x0[d.c[q]];
-          if (NX == 2) prod[NX - 1][k] = d.v[q] * x1[d.c[q]];
+          sv[k] = d.v[q];
+          sc[k] = d.c[q];
         }
       }
       __syncthreads();
@@ -390,9 +406,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
       if (grp < nr) {
         const int s = sptr[grp], e = sptr[grp + 1];
         T a0 = 0, a1 = 0;
+#pragma unroll 2
         for (int k = s + lane; k < e; k += tpr) {
-          a0 += prod[0][k];
-          if (NX == 2) a1 += prod[NX - 1][k];
+          const T v = sv[k];
+          const int c = sc[k];
+          a0 = mul_add_rn(a0, v, x0[c]);
+          if (NX == 2) a1 = mul_add_rn(a1, v, x1[c]);
         }
         for (int off = tpr >> 1; off > 0; off >>= 1) {
           a0 += __shfl_xor(a0, off, 64);
@@ -409,12 +428,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
 }
 
 
-// Wavefront CSR-stream: the scheme of k_spmv_stream with every wavefront
+// Wavefront CSR-stream: the scheme of k_spmv_stream (values and column
+// indices staged in LDS, one row per lane for short rows) with every wavefront
 // working alone on tiles of at most kWaveRows rows and kWaveNnz entries
 // (8 per lane), so no workgroup barrier sits in the tile loop and one slow
 // gather stalls only its own wavefront.  A wavefront's LDS instructions
 // execute in order, so wavefront-scope fences (which keep the compiler from
-// moving LDS accesses across them) order its product writes before its row
+// moving LDS accesses across them) order its staging writes before its row
 // sums and its sums before the next tile's writes.
 constexpr int kWaveNnz = 512;
 constexpr int kWaveRows = 64;
@@ -456,14 +476,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
   st.count();
   epi.prep();
   constexpr int kW = kBlock / 64;
-  __shared__ T prod[NX][kW][kWaveNnz];
+  __shared__ T sv_all[kW][kWaveNnz];
+  __shared__ int32_t sc_all[kW][kWaveNnz];
   __shared__ int32_t sptr_all[kW][kWaveRows + 1];
   double acc[NV > 0 ? NV : 1];
 #pragma unroll
   for (int i = 0; i < (NV > 0 ? NV : 1); ++i) acc[i] = 0.0;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  T* p0 = prod[0][w];
-  T* p1 = prod[NX - 1][w];
+  T* sv = sv_all[w];
+  int32_t* sc = sc_all[w];
   int32_t* sptr = sptr_all[w];
   RowBlocks rbs{ntiles};
   int64_t t, end, step;
@@ -507,8 +528,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
       for (int q = 0; q < kWaveUnroll; ++q) {
         const int k = lane + q * 64;
         if (k < cnt) {
-          p0[k] = d.v[q] * x0[d.c[q]];
-          if (NX == 2) p1[k] = d.v[q] * x1[d.c[q]];
+          sv[k] = d.v[q];
+          sc[k] = d.c[q];
         }
       }
       wave_sync();
@@ -522,9 +543,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
       T a0 = 0, a1 = 0;
       if (grp < nr) {
         const int s = sptr[grp], e = sptr[grp + 1];
+#pragma unroll 4
         for (int k = s + sub; k < e; k += tpr) {
-          a0 += p0[k];
-          if (NX == 2) a1 += p1[k];
+          const T v = sv[k];
+          const int c = sc[k];
+          a0 = mul_add_rn(a0, v, x0[c]);
+          if (NX == 2) a1 = mul_add_rn(a1, v, x1[c]);
         }
       }
       for (int off = tpr >> 1; off > 0; off >>= 1) {
