@@ -252,20 +252,20 @@ def suite(ctx, a):
         if spd:
             # forward and adjoint to 1e-14: the FD comparison at 1e-6 needs
             # gradients accurate well below 1e-6 relative
+            # forward VPCG (FP32 inner CG), backward CG + SDDMM; FD solves by CG
             fwd = dict(outer_eps=1e-14, outer_maxloop=20000, inner_maxloop=10, inner_precision=32)
+            cg = dict(outer_eps=1e-14, outer_maxloop=100000)
             sysm = AG.SparseSystem(ctx, P["ptr"], P["col"], spd=True, solver="vpcg", inner="cg", fp32=True,
-                                   params=fwd, adjoint_solver="vpcg", adjoint_inner="cg",
-                                   adjoint_params=dict(fwd))
-            fd_solver, fd_inner = "vpcg", "cg"
-            fd_params = dict(outer_eps=1e-14, outer_maxloop=20000, inner_maxloop=10, inner_precision=32)
+                                   params=fwd, adjoint_solver="cg", adjoint_params=dict(cg))
+            fd_solver, fd_inner, fd_params = "cg", "NO", dict(cg)
         else:
+            # forward mixed VPGMRES, backward FP64 GMRES on the explicit A^T
             fwd = dict(outer_eps=1e-14, outer_maxloop=20000, outer_restart=21, inner_maxloop=11,
                        inner_restart=12, inner_precision=32)
             sysm = AG.SparseSystem(ctx, P["ptr"], P["col"], spd=False, solver="vpgmres", inner="gmres",
-                                   fp32=True, params=fwd, adjoint_solver="vpgmres", adjoint_inner="gmres",
-                                   adjoint_params=dict(fwd))
-            fd_solver, fd_inner = "vpgmres", "gmres"
-            fd_params = dict(fwd)
+                                   fp32=True, params=fwd, adjoint_solver="gmres",
+                                   adjoint_params=dict(outer_eps=1e-14, outer_maxloop=100000, outer_restart=21))
+            fd_solver, fd_inner, fd_params = "vpgmres", "gmres", dict(fwd)
         vals = torch.from_numpy(P["val"]).cuda().requires_grad_(True)
         bb = b.clone().requires_grad_(True)
         torch.cuda.synchronize()
@@ -279,22 +279,27 @@ def suite(ctx, a):
         t2 = time.perf_counter()
         rec = {"suite": "adjoint", "config": cfg, "n": n, "nnz": nnz, "forward_s": round(t1 - t0, 4),
                "backward_s": round(t2 - t1, 4), "forward_iters": sysm.last_forward["iters"],
+               "forward_relres": sysm.last_forward["relres_true"],
                "adjoint_iters": sysm.last_backward["iters"],
                "adjoint_relres": sysm.last_backward["relres_true"]}
         # central FD along seeded directions (symmetric edge perturbations for the SPD case)
         rng = np.random.default_rng(rng_dirs[cfg])
         errs = []
         gv = vals.grad
+        fdx = AG.CentralFD(ctx, P["ptr"], P["col"], vals.detach(), b, xt, 1e-6, fd_solver, fd_inner,
+                           params=fd_params, spd=spd)
+        t3 = time.perf_counter()
         for _ in range(a.fd_dirs):
             if spd:
                 dk = rng.standard_normal(len(P["kappa"]))
                 dv = torch.from_numpy(gen.aniso2d_values(P["grid"], P["grid"], dk)).cuda()
             else:
                 dv = torch.from_numpy(rng.standard_normal(nnz)).cuda()
-            fd = AG.central_fd(ctx, P["ptr"], P["col"], vals.detach(), dv, b, xt, 1e-6, fd_solver, fd_inner,
-                               params=fd_params, spd=spd, naive=False)
+            fd = fdx(dv)
             ad = float(gv @ dv)
             errs.append(abs(fd - ad) / abs(ad))
+        fdx.close()
+        rec["fd_seconds"] = round(time.perf_counter() - t3, 2)
         rec["fd_dirs"] = a.fd_dirs
         rec["fd_max_rel_err"] = max(errs) if errs else None
         rec["fd_median_rel_err"] = float(np.median(errs)) if errs else None

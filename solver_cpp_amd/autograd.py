@@ -75,40 +75,59 @@ class KrylovSolve(torch.autograd.Function):
         return gv, gb, None
 
 
-def central_fd(ctx, ptr, col, vals, dvals, b, x_target, eps, solver, inner="NO", params=None, spd=False,
-               naive=True):
+class CentralFD:
     """Central difference (L(v + eps d) - L(v - eps d)) / (2 eps) of
-    L = 0.5 ||x(v) - x_target||^2, evaluated without cancellation.
+    L = 0.5 ||x(v) - x_target||^2 along directions d, evaluated without
+    cancellation.
 
     With A+ = A(v + eps d), A- = A(v - eps d) and D = A(d) (same pattern):
     x+ - x- = -2 eps A+^{-1} D x-  (exact identity), so with u = A+^{-1} D x-
     and x+ = x- - 2 eps u the difference quotient is
         -(u . (x- - x_target)) + eps (u . u),
     computed from two FP64 solves whose errors do not cancel catastrophically.
-    Returns (fd, naive_fd) where naive_fd subtracts the two losses directly
-    (a third solve); with naive=False returns fd only.
+    The three matrices are built once; each direction refreshes their values
+    on the device.
     """
-    params = dict(params or {})
-    v = vals.detach().double().cpu().numpy()
-    d = dvals.detach().double().cpu().numpy()
-    fp32 = params.get("inner_precision", 64) == 32
-    Am = api.Matrix(ctx, ptr, col, v - eps * d, transpose=not spd, spd=spd, fp32=fp32)
-    Ap = api.Matrix(ctx, ptr, col, v + eps * d, transpose=not spd, spd=spd, fp32=fp32)
-    D = api.Matrix(ctx, ptr, col, d)
-    try:
-        xm, rm, _ = api.solve(ctx, Am, b, solver, inner, history=False, **params)
-        Dx = D.spmv(xm)
-        u, ru, _ = api.solve(ctx, Ap, Dx, solver, inner, history=False, **params)
-        fd = float(-(u @ (xm - x_target)) + eps * (u @ u))
+
+    def __init__(self, ctx, ptr, col, vals, b, x_target, eps, solver, inner="NO", params=None, spd=False):
+        self.ctx, self.b, self.xt, self.eps = ctx, b, x_target, eps
+        self.solver, self.inner, self.params = solver, inner, dict(params or {})
+        self.vals = torch.as_tensor(vals).detach().double().cuda().contiguous()
+        fp32 = self.params.get("inner_precision", 64) == 32
+        v = self.vals.cpu().numpy()
+        self.Am = api.Matrix(ctx, ptr, col, v, transpose=not spd, spd=spd, fp32=fp32)
+        self.Ap = api.Matrix(ctx, ptr, col, v, transpose=not spd, spd=spd, fp32=fp32)
+        self.D = api.Matrix(ctx, ptr, col, v)
+
+    def __call__(self, dvals, naive=False):
+        d = torch.as_tensor(dvals).detach().double().cuda().contiguous()
+        self.Am.update_values((self.vals - self.eps * d).contiguous())
+        self.Ap.update_values((self.vals + self.eps * d).contiguous())
+        self.D.update_values(d)
+        kw = dict(history=False, **self.params)
+        xm, rm, _ = api.solve(self.ctx, self.Am, self.b, self.solver, self.inner, **kw)
+        u, ru, _ = api.solve(self.ctx, self.Ap, self.D.spmv(xm), self.solver, self.inner, **kw)
         for r in (rm, ru):
             if r["status"] != 0:
                 raise RuntimeError(f"FD solve did not converge: {r}")
+        fd = float(-(u @ (xm - self.xt)) + self.eps * (u @ u))
         if not naive:
             return fd
-        xp, rp, _ = api.solve(ctx, Ap, b, solver, inner, history=False, **params)
-        nv = float((0.5 * ((xp - x_target) ** 2).sum() - 0.5 * ((xm - x_target) ** 2).sum()) / (2 * eps))
+        xp, rp, _ = api.solve(self.ctx, self.Ap, self.b, self.solver, self.inner, **kw)
+        nv = float((0.5 * ((xp - self.xt) ** 2).sum() - 0.5 * ((xm - self.xt) ** 2).sum()) / (2 * self.eps))
         return fd, nv
+
+    def close(self):
+        for m in (self.Am, self.Ap, self.D):
+            m.close()
+
+
+def central_fd(ctx, ptr, col, vals, dvals, b, x_target, eps, solver, inner="NO", params=None, spd=False,
+               naive=True):
+    """One CentralFD evaluation: (fd, naive_fd) where naive_fd subtracts the
+    two losses directly (a third solve); with naive=False returns fd only."""
+    f = CentralFD(ctx, ptr, col, vals, b, x_target, eps, solver, inner, params, spd)
+    try:
+        return f(dvals, naive=naive)
     finally:
-        Am.close()
-        Ap.close()
-        D.close()
+        f.close()
