@@ -6,12 +6,13 @@
 //  * one generic grid-stride map/reduce kernel (k_map) carries every fused
 //    elementwise update + dot product of the solvers; reductions are
 //    deterministic two-stage sums (fixed grid, fixed per-thread element
-//    sets, fixed shuffle tree, the last workgroup sums the per-workgroup
-//    partials in workgroup order);
-//  * CSR SpMV (k_spmv) assigns L lanes per row (L = 2..64, 64 = one
-//    wavefront per row), stages each workgroup's row pointers in LDS and
-//    reduces a row's lane sums with xor shuffles; row blocks are assigned to
-//    XCDs in contiguous ranges so x gathers hit the XCD's own L2;
+//    sets, fixed shuffle tree; a one-workgroup finisher, k_finish, sums the
+//    per-workgroup partials in a fixed order and runs the scalar step);
+//  * CSR SpMV: CSR-stream tiles (k_spmv_stream per workgroup, k_spmv_wave
+//    per wavefront) that stage values and column indices in LDS and sum one
+//    row per lane; a merge-path variant (k_merge_carry + k_spmv_merge); and
+//    a lanes-per-row kernel (k_spmv).  Work is assigned to XCDs in
+//    contiguous ranges so x gathers hit the XCD's own L2;
 //  * solver control flow stays on the device: kernels take a Gate and return
 //    at once when the solve (or the enclosing outer step) has stopped, so
 //    the host can enqueue many iterations between synchronisations;
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restri
 
 // ------------------------------------------------------- generic map ----
 // op.prep() once per thread (load scalars), op.elem(i, acc) per element,
-// op.fin(sums) by one thread of the last workgroup when NV > 0.
+// op.fin(sums) by the finisher (k_finish, launched after it) when NV > 0.
 template <int NV, class Op>
 __global__ __launch_bounds__(kBlock) void k_map(int64_t n, Op op, Gate g, RedArgs ra, StatArgs st) {
   if (g.off()) return;
@@ -562,50 +563,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
     nr0 = mr0; nr1 = mr1; nj0 = mj0; nj1 = mj1;
   }
   if constexpr (NV > 0) block_store<NV>(acc, ra);
-}
-// Two products with one pass over the matrix: y1 = A x1, y2 = A x2
-// (matrix-powers basis of the k-skip solvers).
-template <class T, int L>
-__global__ __launch_bounds__(kBlock) void k_spmv2(int64_t n, const int32_t* __restrict__ ptr,
-                                                  const int32_t* __restrict__ col,
-                                                  const T* __restrict__ val, const T* __restrict__ x1,
-                                                  const T* __restrict__ x2, T* __restrict__ y1,
-                                                  T* __restrict__ y2, Gate g, StatArgs st) {
-  if (g.off()) return;
-  st.count();
-  constexpr int RPB = kBlock / L;
-  __shared__ int32_t sptr[RPB + 1];
-  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
-  RowBlocks rbs{(n + RPB - 1) / RPB};
-  int64_t rb, end, step;
-  rbs.range(rb, end, step);
-  for (; rb < end; rb += step) {
-    const int64_t row0 = rb * RPB;
-    __syncthreads();
-    if (threadIdx.x <= RPB) {
-      int64_t r = row0 + threadIdx.x;
-      sptr[threadIdx.x] = ptr[r < n ? r : n];
-    }
-    __syncthreads();
-    const int64_t row = row0 + grp;
-    const int s = sptr[grp], e = sptr[grp + 1];
-    T s1 = 0, s2 = 0;
-    for (int j = s + lane; j < e; j += L) {
-      const T a = val[j];
-      const int c = col[j];
-      s1 += a * x1[c];
-      s2 += a * x2[c];
-    }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) {
-      s1 += __shfl_xor(s1, off, L);
-      s2 += __shfl_xor(s2, off, L);
-    }
-    if (lane == 0 && row < n) {
-      y1[row] = s1;
-      y2[row] = s2;
-    }
-  }
 }
 
 // out[j] = alpha * u[row(j)] * v[col(j)] over the pattern (SDDMM).
