@@ -1,10 +1,14 @@
 """GPU solvers vs the CPU oracle on the same seeded inputs (through the C ABI).
 
-Parity bar: the reported iteration count (`loop =`) within +-1 of the
-oracle's, the same exit status, and x within a relative distance that the
-rounding-order difference allows (the GPU reduces in a different order and
-with FMA); residual histories agree to the same tolerance.
+Parity bar (FP64 solvers): the reported iteration count (`loop =`) within
++-1 of the oracle's, the same exit status, x within 1e-10 relative, and the
+same history indices.  Solvers with an FP32 inner solve: x within 1e-9.
+(Measured on MI355X: identical iteration counts in every case here, x within
+1.2e-13.)  KS_PARITY_REPORT=<file> appends the measured distances.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -19,11 +23,22 @@ def _mat(ctx, ptr, col, val, **kw):
     return api.Matrix(ctx, ptr, col, val, **kw)
 
 
-def _run(ctx, oracle, ptr, col, val, b, outer, inner="NO", xtol=1e-8, it_tol=1, **kw):
+def _report(outer, inner, kw, r, ro, xdiff):
+    # KS_PARITY_REPORT=<file>: append the measured distances (one JSON line)
+    path = os.environ.get("KS_PARITY_REPORT")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": os.environ.get("PYTEST_CURRENT_TEST", "").split(" ")[0], "outer": outer,
+                                "inner": inner, "iters": r["iters"], "oracle_iters": ro["iters"],
+                                "x_rel_diff": xdiff, "prec": kw.get("inner_precision", 64)}) + "\n")
+
+
+def _run(ctx, oracle, ptr, col, val, b, outer, inner="NO", xtol=1e-10, it_tol=1, **kw):
     from solver_cpp_amd import api
     A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
     x, r, h = api.solve(ctx, A, b, outer, inner, **kw)
     xo, ro, ho = oracle.solve(ptr, col, val, b, outer, inner, **kw)
+    _report(outer, inner, kw, r, ro, float(np.linalg.norm(x - xo) / max(np.linalg.norm(xo), 1e-300)))
     assert r["status"] == ro["status"], (r, ro)
     assert abs(r["iters"] - ro["iters"]) <= it_tol, (r["iters"], ro["iters"])
     if r["status"] == 0:
@@ -100,7 +115,7 @@ def test_vpgmres_gmres_inner(ctx, oracle, prec):
     ptr, col, val, b, xt = P.convdiff3d(14)
     x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "vpgmres", "gmres", outer_eps=1e-10,
                                outer_maxloop=500, outer_restart=21, inner_maxloop=11, inner_restart=12,
-                               inner_precision=prec, xtol=1e-7)
+                               inner_precision=prec, xtol=1e-9)
     assert r["status"] == 0 and r["relres_true"] < 2e-10
     assert r["inner_steps"] > 0
 
@@ -121,7 +136,7 @@ def test_vpgmres_ref_last_term(ctx, oracle):
 def test_vpcg_cg_inner(ctx, oracle, prec):
     ptr, col, val, b, xt = P.convdiff3d(14, pe=0.0)
     x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "vpcg", "cg", outer_eps=1e-10, outer_maxloop=500,
-                               inner_maxloop=10, inner_precision=prec, xtol=1e-7)
+                               inner_maxloop=10, inner_precision=prec, xtol=1e-9)
     assert r["status"] == 0
 
 
@@ -158,7 +173,7 @@ def test_gcr_maxloop(ctx, oracle):
 def test_vpgcr(ctx, oracle, prec):
     ptr, col, val, b, xt = P.convdiff3d(12)
     _run(ctx, oracle, ptr, col, val, b, "vpgcr", "gmres", outer_eps=1e-10, outer_maxloop=300, outer_restart=20,
-         inner_maxloop=6, inner_restart=7, inner_precision=prec, xtol=1e-7)
+         inner_maxloop=6, inner_restart=7, inner_precision=prec, xtol=1e-9)
 
 
 def test_inner_solver_kinds(ctx, oracle):
@@ -166,7 +181,7 @@ def test_inner_solver_kinds(ctx, oracle):
     ptr, col, val, b, xt = P.convdiff3d(10, pe=0.0)
     for inner in ("cg", "cr", "gcr", "bicg", "gmres"):
         _run(ctx, oracle, ptr, col, val, b, "vpgmres", inner, outer_eps=1e-10, outer_maxloop=200,
-             outer_restart=21, inner_maxloop=5, inner_restart=8, xtol=1e-7)
+             outer_restart=21, inner_maxloop=5, inner_restart=8, xtol=1e-9)
 
 
 def test_zero_rhs(ctx):
@@ -225,9 +240,10 @@ def test_solvers_on_every_spmv_layout(ctx, oracle, layout):
         x0, r0, _ = api.solve(ctx, A0, b, outer, inner, **kw)
         x1, r1, _ = api.solve(ctx, A1, b, outer, inner, **kw)
         assert r0["status"] == 0 and r1["status"] == 0, (outer, r0, r1)
-        tol = 3 if kw.get("inner_precision") == 32 else 1
+        mixed = kw.get("inner_precision") == 32
+        tol = 3 if mixed else 1
         assert abs(r0["iters"] - r1["iters"]) <= tol, (outer, r0["iters"], r1["iters"])
-        assert np.linalg.norm(x1 - x0) <= 1e-7 * np.linalg.norm(x0), outer
+        assert np.linalg.norm(x1 - x0) <= (1e-7 if mixed else 1e-10) * np.linalg.norm(x0), outer
         xo, ro, _ = oracle.solve(ptr, col, val, b, outer, inner, **kw)
         assert abs(r1["iters"] - ro["iters"]) <= tol, (outer, r1["iters"], ro["iters"])
 
@@ -258,6 +274,6 @@ def test_cli_solver_end_to_end(tmp_path, oracle):
         xv = np.loadtxt(work / "output" / "GMRES_xvec.txt")[:, 1]
         his = np.loadtxt(work / "output" / "GMRES_his.txt")
         assert np.array_equal(his[:, 0], ho[:, 0])  # same check indices as the oracle
-        assert np.linalg.norm(xv - xo) <= 1e-8 * np.linalg.norm(xo)
+        assert np.linalg.norm(xv - xo) <= 1e-10 * np.linalg.norm(xo)
         outs.append(xv)
     assert np.array_equal(outs[0], outs[1])
