@@ -63,11 +63,10 @@ typedef enum ks_status {
 #define KS_MAT_SPD      16u /* caller asserts A is symmetric positive definite */
 #define KS_MAT_MERGE    32u /* force the merge-path SpMV for this matrix */
 #define KS_MAT_VECTOR   64u /* force the lanes-per-row vector SpMV */
-/* CSR-stream tiles: by default the FP64 SpMV works on wavefront tiles (64
- * rows / 512 entries) and the FP32 SpMV on workgroup tiles (256 rows / 2048
- * entries); these flags force one kind for both */
-#define KS_MAT_WG_TILES 128u   /* workgroup tiles for both */
-#define KS_MAT_WAVE_TILES 512u /* wavefront tiles for both */
+/* CSR-stream tiles: by default the SpMV works on workgroup tiles (256 rows /
+ * 2047 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 511) */
+#define KS_MAT_WG_TILES 128u   /* workgroup tiles (the default) */
+#define KS_MAT_WAVE_TILES 512u /* wavefront tiles */
 #define KS_MAT_BORROW   256u /* with KS_PTR_DEVICE: use the caller's ptr/col/val
                                 in place (no copy); they must outlive the
                                 matrix, and values changed in place are made

@@ -172,9 +172,8 @@ class Context:
 class Matrix:
     """CSR matrix in HBM (collection's Cval/Ccol/Cptr + CTval/... for A^T)."""
 
-    # SpMV layouts: CSR-stream (default: FP64 on wavefront tiles, FP32 on
-    # workgroup tiles), CSR-stream with workgroup or wavefront tiles for both
-    # value sizes, lanes-per-row vector kernel, merge-path
+    # SpMV layouts: CSR-stream on workgroup tiles (default; "wg" names it
+    # explicitly) or wavefront tiles, lanes-per-row vector kernel, merge-path
     LAYOUTS = {"stream": 0, "wg": _abi.KS_MAT_WG_TILES, "wave": _abi.KS_MAT_WAVE_TILES,
                "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE}
 

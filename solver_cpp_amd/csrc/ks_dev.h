@@ -166,15 +166,14 @@ struct TileSet {
 };
 
 // SpMV plan of one CSR matrix: CSR-stream tiles (kind 0), L lanes per row
-// (kind 1) or merge-path segments (kind 2).  Kinds 0 and 1 keep wavefront
-// tiles (k_spmv_wave) and/or workgroup tiles (k_spmv_stream); by default the
-// FP64 SpMV uses the wavefront tiles and the FP32 SpMV the workgroup tiles
-// (the faster of the two for each value size on MI355X), the k-skip
-// two-input SpMV the FP64 choice.  For kind 2, `seg` holds the segment
-// start points (row, entry) of the merge path.
+// (kind 1) or merge-path segments (kind 2).  Kinds 0 and 1 keep workgroup
+// tiles (k_spmv_stream, the default: the faster kind for both value sizes on
+// MI355X) and/or wavefront tiles (k_spmv_wave); the k-skip two-input SpMV
+// uses the FP64 choice.  For kind 2, `seg` holds the segment start points
+// (row, entry) of the merge path.
 struct Plan {
   int kind = 0;
-  bool wg64 = false, wg32 = true;  // tile set of the FP64 / FP32 SpMV
+  bool wg64 = true, wg32 = true;  // tile set of the FP64 / FP32 SpMV
   int L = 8;
   TileSet wave, wg, seg;
   DBuf<double> carry;  // merge path: 2 * segments carries (FP64 or FP32)
@@ -469,8 +468,7 @@ inline void tile_set(ks_ctx* c, TileSet& ts, const int32_t* hp, int64_t n, int64
   upload_tiles(c, ts, t, z);
 }
 
-// tiles: 0 default (FP64 wavefront, FP32 workgroup tiles), 1 workgroup tiles
-// for both, 2 wavefront tiles for both.
+// tiles: 0 or 1 workgroup tiles (the default), 2 wavefront tiles.
 inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t nnz, bool vector, int tiles,
                       bool merge = false) {
   pl.L = 2;
@@ -484,8 +482,7 @@ inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t
     upload_tiles(c, pl.seg, t, z);
     return;
   }
-  pl.wg64 = tiles == 1;
-  pl.wg32 = tiles != 2;
+  pl.wg64 = pl.wg32 = tiles != 2;
   if (!pl.wg64 || !pl.wg32) tile_set(c, pl.wave, hp, n, kWaveRows, kWaveNnz);
   if (pl.wg64 || pl.wg32) tile_set(c, pl.wg, hp, n, kTileRows, kTileNnz);
 }

@@ -490,6 +490,9 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   if (flags & KS_MAT_BORROW) {
     KS_CHECK(flags & KS_PTR_DEVICE, KS_ERR_BAD_INPUT, "KS_MAT_BORROW needs device arrays (KS_PTR_DEVICE)");
     KS_CHECK(nnz > 0, KS_ERR_BAD_INPUT, "KS_MAT_BORROW needs nnz > 0");
+    // the SpMV reads values and column indices in aligned pairs
+    KS_CHECK((uintptr_t)col % 16 == 0 && (uintptr_t)val % 16 == 0, KS_ERR_BAD_INPUT,
+             "KS_MAT_BORROW needs 16-byte aligned col and val arrays");
     m->ptr.borrow(const_cast<int32_t*>(ptr), n + 1);
     m->col.borrow(const_cast<int32_t*>(col), nnz);
     m->val.borrow(const_cast<double*>(val), nnz);

@@ -269,9 +269,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
 // indices of the workgroup's next tile are already in flight (eight
 // independent loads per thread).  NX = 2 forms A x0 and A x1 from one pass
 // over the matrix (the k-skip matrix-powers basis).
-constexpr int kTileNnz = 2048;
+// Values and column indices are read as pairs (entries e, e+1 with e even:
+// 8-byte column loads, 8- or 16-byte value loads).  A tile's pairs cover the
+// even-aligned window [j0 & ~1, (j0 & ~1) + window), so a tile holds at most
+// window - 1 entries; the matrix's last entry, when it starts a pair, is read
+// alone, so no load passes nnz.
+constexpr int kTileWin = 2048;
+constexpr int kTileNnz = kTileWin - 1;
 constexpr int kTileRows = 256;
-constexpr int kTileUnroll = kTileNnz / kBlock;
 
 template <class T>
 __device__ inline T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
@@ -285,25 +290,63 @@ __device__ inline T mul_add_rn(T a, T v, T x) {
 }
 
 template <class T>
-struct TileData {
-  T v[kTileUnroll];
-  int c[kTileUnroll];
-  int32_t pr;  // row pointer of row (r0 + tid)
-  __device__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                       const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1) {
-    const int cnt = j1 - j0;
-    if (cnt > kTileNnz) return;  // long row: loaded in place
+struct Vec2 {
+  typedef T type __attribute__((ext_vector_type(2)));
+};
+
+// One thread's share of a tile: NTH threads (a workgroup or a wavefront)
+// cover a window of WIN entries, thread t taking the pairs at 2t + 2 NTH q.
+template <class T, int NTH, int WIN>
+struct PairTile {
+  static constexpr int kPairs = WIN / (2 * NTH);
+  T v[2 * kPairs];
+  int c[2 * kPairs];
+  int32_t pr;  // row pointer of row r0 + t
+  __device__ __forceinline__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                       const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1,
+                                       int32_t nnz, int t) {
+    if (j1 - j0 > WIN - 1) return;  // long row: loaded in place
+    const int32_t a0 = j0 & ~1;
 #pragma unroll
-    for (int q = 0; q < kTileUnroll; ++q) {
-      const int k = threadIdx.x + q * kBlock;
-      if (k < cnt) {
-        v[q] = ld_stream(val + j0 + k);
-        c[q] = ld_stream(col + j0 + k);
+    for (int q = 0; q < kPairs; ++q) {
+      const int32_t e = a0 + 2 * t + 2 * NTH * q;
+      if (e < j1) {
+        if (e + 1 < nnz) {
+          const typename Vec2<T>::type vv = ld_stream(reinterpret_cast<const typename Vec2<T>::type*>(val + e));
+          const typename Vec2<int>::type cc = ld_stream(reinterpret_cast<const typename Vec2<int>::type*>(col + e));
+          v[2 * q] = vv[0];
+          v[2 * q + 1] = vv[1];
+          c[2 * q] = cc[0];
+          c[2 * q + 1] = cc[1];
+        } else {
+          // both halves written on both paths, so the array stays in registers
+          v[2 * q] = ld_stream(val + e);
+          c[2 * q] = ld_stream(col + e);
+          v[2 * q + 1] = 0;
+          c[2 * q + 1] = 0;
+        }
       }
     }
-    pr = (int)threadIdx.x < r1 - r0 ? ptr[r0 + threadIdx.x] : 0;
+    pr = t < r1 - r0 ? ptr[r0 + t] : 0;
+  }
+  // the tile's entries into sv / sc at their offsets from j0
+  __device__ __forceinline__ void stage(T* sv, int32_t* sc, int32_t j0, int32_t j1, int t) const {
+    const int32_t a0 = j0 & ~1;
+#pragma unroll
+    for (int q = 0; q < kPairs; ++q) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int32_t k = a0 + 2 * t + 2 * NTH * q + h - j0;
+        if (k >= 0 && k < j1 - j0) {
+          sv[k] = v[2 * q + h];
+          sc[k] = c[2 * q + h];
+        }
+      }
+    }
   }
 };
+template <class T>
+using TileData = PairTile<T, kBlock, kTileWin>;
 
 // Register budget of the CSR-stream kernels as waves/SIMD: FP32 64 VGPRs
 // (8 waves) for a plain SpMV and 72 (7 waves) with a fused reduction, FP64
@@ -326,8 +369,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
   if (g.off()) return;
   st.count();
   epi.prep();
-  __shared__ T sv[kTileNnz];
-  __shared__ int32_t sc[kTileNnz];
+  __shared__ T sv[kTileWin];
+  __shared__ int32_t sc[kTileWin];
   __shared__ int32_t sptr[kTileRows + 1];
   __shared__ double slong[NX][kBlock / 64];
   double acc[NV > 0 ? NV : 1];
@@ -337,11 +380,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
   RowBlocks rbs{ntiles};
   int64_t t, end, step;
   rbs.range(t, end, step);
+  const int32_t nnz = tnz[ntiles];
   int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
   TileData<T> d;
   if (t < end) {
     r0 = tiles[t]; r1 = tiles[t + 1]; j0 = tnz[t]; j1 = tnz[t + 1];
-    d.load(ptr, col, val, r0, r1, j0, j1);
+    d.load(ptr, col, val, r0, r1, j0, j1, nnz, tid);
   }
   // meta of the next tile: loaded one tile ahead, after the data prefetch,
   // so waiting for the prefetched columns never waits for these loads
@@ -382,23 +426,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
         }
         epi.row(r0, a0, a1, acc);
       }
-      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1);
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, nnz, tid);
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       __syncthreads();
     } else {
       if (tid < nr) sptr[tid] = d.pr - j0;
       if (tid == 0) sptr[nr] = cnt;
-#pragma unroll
-      for (int q = 0; q < kTileUnroll; ++q) {
-        const int k = tid + q * kBlock;
-        if (k < cnt) {
-          sv[k] = d.v[q];
-          sc[k] = d.c[q];
-        }
-      }
+      d.stage(sv, sc, j0, j1, tid);
       __syncthreads();
       // in flight during the sums: the next tile's data, then the meta after it
-      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1);
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, nnz, tid);
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       // more than one thread per row only for rows averaging > 8 entries
       int tpr = 1;
@@ -437,9 +474,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
 // execute in order, so wavefront-scope fences (which keep the compiler from
 // moving LDS accesses across them) order its staging writes before its row
 // sums and its sums before the next tile's writes.
-constexpr int kWaveNnz = 512;
+constexpr int kWaveWin = 512;
+constexpr int kWaveNnz = kWaveWin - 1;
 constexpr int kWaveRows = 64;
-constexpr int kWaveUnroll = kWaveNnz / 64;
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -448,25 +485,7 @@ __device__ inline void wave_sync() {
 }
 
 template <class T>
-struct WaveTile {
-  T v[kWaveUnroll];
-  int c[kWaveUnroll];
-  int32_t pr;  // row pointer of row (r0 + lane)
-  __device__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                       const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1, int lane) {
-    const int cnt = j1 - j0;
-    if (cnt > kWaveNnz) return;  // long row: loaded in place
-#pragma unroll
-    for (int q = 0; q < kWaveUnroll; ++q) {
-      const int k = lane + q * 64;
-      if (k < cnt) {
-        v[q] = ld_stream(val + j0 + k);
-        c[q] = ld_stream(col + j0 + k);
-      }
-    }
-    pr = lane < r1 - r0 ? ptr[r0 + lane] : 0;
-  }
-};
+using WaveTile = PairTile<T, 64, kWaveWin>;
 
 template <class T, int NX, int NV, class Epi>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamWaves<T, NX, NV>))) void k_spmv_wave(
@@ -477,8 +496,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
   st.count();
   epi.prep();
   constexpr int kW = kBlock / 64;
-  __shared__ T sv_all[kW][kWaveNnz];
-  __shared__ int32_t sc_all[kW][kWaveNnz];
+  __shared__ T sv_all[kW][kWaveWin];
+  __shared__ int32_t sc_all[kW][kWaveWin];
   __shared__ int32_t sptr_all[kW][kWaveRows + 1];
   double acc[NV > 0 ? NV : 1];
 #pragma unroll
@@ -490,11 +509,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
   RowBlocks rbs{ntiles};
   int64_t t, end, step;
   rbs.range(t, end, step, w, kW);
+  const int32_t nnz = tnz[ntiles];
   int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
   WaveTile<T> d;
   if (t < end) {
     r0 = tiles[t]; r1 = tiles[t + 1]; j0 = tnz[t]; j1 = tnz[t + 1];
-    d.load(ptr, col, val, r0, r1, j0, j1, lane);
+    d.load(ptr, col, val, r0, r1, j0, j1, nnz, lane);
   }
   int32_t nr0 = 0, nr1 = 0, nj0 = 0, nj1 = 0;
   if (t + step < end) {
@@ -514,7 +534,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
         s0 += a * x0[c];
         if (NX == 2) s1 += a * x1[c];
       }
-      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, lane);
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, nnz, lane);
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
@@ -525,17 +545,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
     } else {
       if (lane < nr) sptr[lane] = d.pr - j0;
       if (lane == 0) sptr[nr] = cnt;
-#pragma unroll
-      for (int q = 0; q < kWaveUnroll; ++q) {
-        const int k = lane + q * 64;
-        if (k < cnt) {
-          sv[k] = d.v[q];
-          sc[k] = d.c[q];
-        }
-      }
+      d.stage(sv, sc, j0, j1, lane);
       wave_sync();
       // in flight during the sums: the next tile's data, then the bounds after it
-      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, lane);
+      if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, nnz, lane);
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       // more than one lane per row only for rows averaging > 8 entries
       int tpr = 1;
