@@ -804,8 +804,17 @@ struct FinProj {
   double* hout;
   int ldh, nv;
   const double* vn;
+  // in chunks of 8: the chunk's norms are loaded before its stores (hout and
+  // vn do not overlap), one memory round trip per chunk
   __device__ void fin(const double* s) const {
-    for (int v = 0; v < nv; ++v) hout[(int64_t)v * ldh] = vn ? s[v] / vn[v] : s[v];
+    for (int v0 = 0; v0 < nv; v0 += 8) {
+      double d[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) d[j] = (vn && v0 + j < nv) ? vn[v0 + j] : 1.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (v0 + j < nv) hout[(int64_t)(v0 + j) * ldh] = vn ? s[v0 + j] / d[j] : s[v0 + j];
+    }
   }
 };
 
@@ -951,16 +960,27 @@ __global__ void k_mark_done(Ctl* c, Gate g) {
 // Givens rotations for column k (gmres.cpp:384-401), then the stop test of
 // step k+1 when it lies in this cycle (gmres.cpp:278-324).  Runs in the
 // finisher of the Gram-Schmidt norm, right after h_{k+1,k} is known.
-// The column is rotated in chunks of 8: the chunk's H entries, cosines and
-// sines are loaded together (H, cs, sn, e do not overlap), so a step costs
-// one memory round trip per 8 rotations instead of one per rotation.
-__device__ inline void givens_step(Ctl* c, int k) {
+// GMRES step k after the Gram-Schmidt norm h = ||w||: H[k+1][k] = h, the
+// Givens rotations of column k (gmres.cpp:384-399), the residual estimate
+// |e_{k+1}| / ||b|| and the stop test (gmres.cpp:278, the `count+1 >=
+// maxloop` rule at :286).  Runs on one thread, so it is written as loads,
+// then arithmetic, then stores: every control field and array entry it needs
+// is read before its first store (H, cs, sn, e, vn and the control block do
+// not overlap), and the column is rotated in chunks of 8 whose entries are
+// loaded together - a few memory round trips per step instead of one per
+// field and per rotation.
+__device__ inline void gm_norm_step(Ctl* c, int k, double h) {
   double* __restrict__ H = c->H;
   double* __restrict__ cs = c->cs;
   double* __restrict__ sn = c->sn;
   double* __restrict__ e = c->e;
-  const int ld = c->ld;
-  const double hk1 = H[(k + 1) * ld + k];
+  double* __restrict__ vn = c->vn;
+  const int ld = c->ld, R = c->R;
+  const long long count = c->count, steps = c->steps, tsteps = c->tsteps, maxloop = c->maxloop;
+  const long long nhist = c->nhist, hist_cap = c->hist_cap;
+  double* hist = c->hist;
+  int* host_stop = c->host_stop;
+  const double bnorm = c->bnorm, eps = c->eps;
   const double ek = e[k];
   double t1 = H[k];  // H[0][k], then the partly rotated H[i][k]
   for (int i0 = 0; i0 < k; i0 += 8) {
@@ -981,31 +1001,39 @@ __device__ inline void givens_step(Ctl* c, int k) {
       }
     }
   }
-  // t1 is now the rotated H[k][k]
+  // t1 is now the rotated H[k][k]; H[k+1][k] = h is annihilated
   const double hk = t1;
-  const double alpha = sqrt(hk * hk + hk1 * hk1);
-  const double ck = hk / alpha, sk = -hk1 / alpha;
+  const double alpha = sqrt(hk * hk + h * h);
+  const double ck = hk / alpha, sk = -h / alpha;
+  const double ek1 = sk * ek;
   cs[k] = ck;
   sn[k] = sk;
-  e[k + 1] = sk * ek;
+  e[k + 1] = ek1;
   e[k] = ck * ek;
-  H[k * ld + k] = ck * hk - sk * hk1;
+  H[k * ld + k] = ck * hk - sk * h;
   H[(k + 1) * ld + k] = 0.0;
-  c->count++;
-  c->steps++;
-  c->tsteps++;
-  if (k + 1 < c->R - 1) {
-    const double err = fabs(c->e[k + 1]) / c->bnorm;
-    ctl_hist(c, (double)(c->count + 1), err);
-    if (c->count + 1 >= c->maxloop) {
+  vn[k + 1] = h;
+  c->vscale = h;
+  c->count = count + 1;
+  c->steps = steps + 1;
+  c->tsteps = tsteps + 1;
+  if (k + 1 < R - 1) {
+    const double err = fabs(ek1) / bnorm;
+    c->err = err;
+    if (hist && nhist < hist_cap) {
+      hist[2 * nhist] = (double)(count + 2);
+      hist[2 * nhist + 1] = err;
+    }
+    c->nhist = nhist + 1;
+    int st = -1;
+    if (count + 2 >= maxloop) st = 2;
+    else if (err <= eps) st = 0;
+    else if (!finite(err)) st = 3;
+    if (st >= 0) {
       c->ksz = k + 1;
-      ctl_stop(c, 2);
-    } else if (err <= c->eps) {
-      c->ksz = k + 1;
-      ctl_stop(c, 0);
-    } else if (!finite(err)) {
-      c->ksz = k + 1;
-      ctl_stop(c, 3);
+      c->stop = 1;
+      c->status = st;
+      if (host_stop) *(volatile int*)host_stop = 1;
     }
   }
 }

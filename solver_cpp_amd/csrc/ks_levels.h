@@ -275,13 +275,7 @@ struct OpGmFirstFin : OpGmFirst<T> {
 struct FinGmNorm {
   Ctl* c;
   int k;
-  __device__ void operator()(const double* s) const {
-    const double h = sqrt(s[0]);
-    c->H[(k + 1) * c->ld + k] = h;
-    c->vscale = h;
-    c->vn[k + 1] = h;
-    givens_step(c, k);
-  }
+  __device__ void operator()(const double* s) const { gm_norm_step(c, k, sqrt(s[0])); }
 };
 
 // Residual/bnorm helper ops for the API entry points.
