@@ -61,7 +61,9 @@ typedef enum ks_status {
 #define KS_MAT_FP32_COPY 4u /* keep an FP32 copy of the values (mixed inner) */
 #define KS_MAT_TRANSPOSE 8u /* build the explicit transposed CSR (BiCG, adjoint) */
 #define KS_MAT_SPD      16u /* caller asserts A is symmetric positive definite */
-#define KS_MAT_MERGE    32u /* force the merge-path SpMV for this matrix */
+#define KS_MAT_MERGE    32u /* merge-path SpMV for this matrix (with no
+                               layout flag it is chosen automatically when the
+                               longest row exceeds 256 entries and 16x the mean) */
 #define KS_MAT_VECTOR   64u /* force the lanes-per-row vector SpMV */
 /* CSR-stream tiles: by default the SpMV works on workgroup tiles (256 rows /
  * 2047 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 511) */

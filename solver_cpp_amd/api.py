@@ -172,12 +172,14 @@ class Context:
 class Matrix:
     """CSR matrix in HBM (collection's Cval/Ccol/Cptr + CTval/... for A^T)."""
 
-    # SpMV layouts: CSR-stream on workgroup tiles (default; "wg" names it
-    # explicitly) or wavefront tiles, lanes-per-row vector kernel, merge-path
-    LAYOUTS = {"stream": 0, "wg": _abi.KS_MAT_WG_TILES, "wave": _abi.KS_MAT_WAVE_TILES,
-               "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE}
+    # SpMV layouts: "auto" (the library's choice: merge-path for heavy-tailed
+    # row lengths, CSR-stream on workgroup tiles otherwise), CSR-stream on
+    # workgroup ("stream" / "wg") or wavefront tiles, lanes-per-row vector
+    # kernel, merge-path
+    LAYOUTS = {"auto": 0, "stream": _abi.KS_MAT_WG_TILES, "wg": _abi.KS_MAT_WG_TILES,
+               "wave": _abi.KS_MAT_WAVE_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE}
 
-    def __init__(self, ctx, ptr, col, val, transpose=False, fp32=False, spd=False, layout="stream",
+    def __init__(self, ctx, ptr, col, val, transpose=False, fp32=False, spd=False, layout="auto",
                  borrow=False):
         self.ctx = ctx
         ap, ac, av = _Arg(ptr, np.int32), _Arg(col, np.int32), _Arg(val, np.float64)

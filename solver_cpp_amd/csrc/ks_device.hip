@@ -112,6 +112,16 @@ int tile_choice(uint32_t flags) {
   return 0;
 }
 
+// Merge path for KS_MAT_MERGE, and by default (no layout flag) for
+// heavy-tailed row lengths: a longest row above 256 entries and 16 times the
+// mean (the Zipf-like case, where it beats CSR-stream tiles).
+bool merge_choice(uint32_t flags, int64_t n, int64_t nnz, int64_t max_row) {
+  if (flags & KS_MAT_MERGE) return true;
+  if (flags & (KS_MAT_VECTOR | KS_MAT_WG_TILES | KS_MAT_WAVE_TILES)) return false;
+  const double mean = n > 0 ? (double)nnz / (double)n : 0.0;
+  return max_row > 256 && (double)max_row > 16.0 * mean;
+}
+
 void build_transpose(ks_mat* m) {
   ks_ctx* c = m->ctx;
   const int64_t n = m->n, nnz = m->nnz;
@@ -163,7 +173,7 @@ void build_transpose(ks_mat* m) {
     KS_HIPCK(hipGetLastError());
   }
   make_plan(c, m->tplan, htp.data(), n, nnz, (m->flags & KS_MAT_VECTOR) != 0, tile_choice(m->flags),
-            (m->flags & KS_MAT_MERGE) != 0);
+            merge_choice(m->flags, n, nnz, mx));
   m->has_t = true;
 }
 
@@ -525,7 +535,7 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
     KS_CHECK(hc[j] >= 0 && hc[j] < n, KS_ERR_BAD_INPUT, "column index out of range at entry " + std::to_string(j));
   m->max_row = mx;
   make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0, tile_choice(flags),
-              (flags & KS_MAT_MERGE) != 0);
+              merge_choice(flags, n, nnz, mx));
   if (flags & KS_MAT_TRANSPOSE) build_transpose(m.get());
   if (flags & KS_MAT_FP32_COPY) ensure_fp32(m.get());
   KS_HIPCK(hipStreamSynchronize(c->stream));
