@@ -122,6 +122,56 @@ bool merge_choice(uint32_t flags, int64_t n, int64_t nnz, int64_t max_row) {
   return max_row > 256 && (double)max_row > 16.0 * mean;
 }
 
+// A^T row segments longer than kSortMax: bitonic sorts of kSortMax-entry
+// chunks, then pairwise merges of sorted runs (width kSortMax, 2 kSortMax,
+// ...) that alternate between A^T's arrays and a flat staging buffer.
+void sort_huge_segments(ks_mat* m, const std::vector<int32_t>& htp, hipStream_t s) {
+  const int64_t n = m->n;
+  std::vector<int32_t> cs, cl;
+  std::vector<int64_t> off{0}, sb, fb;
+  int64_t longest = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    const int64_t a = htp[r], len = htp[r + 1] - a;
+    if (len <= kSortMax) continue;
+    for (int64_t q = 0; q < len; q += kSortMax) {
+      cs.push_back((int32_t)(a + q));
+      cl.push_back((int32_t)std::min<int64_t>(kSortMax, len - q));
+    }
+    sb.push_back(a);
+    fb.push_back(off.back());
+    off.push_back(off.back() + len);
+    longest = std::max(longest, len);
+  }
+  const int32_t nh = (int32_t)sb.size();
+  const int64_t total = off.back();
+  DBuf<int32_t> dcs(cs.size()), dcl(cl.size()), fc(total), fp(total);
+  DBuf<int64_t> doff(off.size()), dsb(nh), dfb(nh);
+  KS_HIPCK(hipMemcpyAsync(dcs.p, cs.data(), sizeof(int32_t) * cs.size(), hipMemcpyHostToDevice, s));
+  KS_HIPCK(hipMemcpyAsync(dcl.p, cl.data(), sizeof(int32_t) * cl.size(), hipMemcpyHostToDevice, s));
+  KS_HIPCK(hipMemcpyAsync(doff.p, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice, s));
+  KS_HIPCK(hipMemcpyAsync(dsb.p, sb.data(), sizeof(int64_t) * nh, hipMemcpyHostToDevice, s));
+  KS_HIPCK(hipMemcpyAsync(dfb.p, fb.data(), sizeof(int64_t) * nh, hipMemcpyHostToDevice, s));
+  k_sort_chunks<<<(unsigned)cs.size(), 1024, 0, s>>>(dcs.p, dcl.p, m->tcol.p, m->tperm.p);
+  KS_HIPCK(hipGetLastError());
+  bool in_flat = false;
+  for (int64_t w = kSortMax; w < longest; w *= 2) {
+    if (in_flat)
+      k_merge_runs<<<grid_map(total), kBlock, 0, s>>>(total, nh, doff.p, dfb.p, dsb.p, w, fc.p, fp.p, m->tcol.p,
+                                                      m->tperm.p);
+    else
+      k_merge_runs<<<grid_map(total), kBlock, 0, s>>>(total, nh, doff.p, dsb.p, dfb.p, w, m->tcol.p, m->tperm.p,
+                                                      fc.p, fp.p);
+    KS_HIPCK(hipGetLastError());
+    in_flat = !in_flat;
+  }
+  if (in_flat) {  // a run as wide as every segment has no partner: a plain copy back
+    k_merge_runs<<<grid_map(total), kBlock, 0, s>>>(total, nh, doff.p, dfb.p, dsb.p, int64_t(1) << 40, fc.p, fp.p,
+                                                    m->tcol.p, m->tperm.p);
+    KS_HIPCK(hipGetLastError());
+  }
+  KS_HIPCK(hipStreamSynchronize(s));
+}
+
 void build_transpose(ks_mat* m) {
   ks_ctx* c = m->ctx;
   const int64_t n = m->n, nnz = m->nnz;
@@ -162,12 +212,10 @@ void build_transpose(ks_mat* m) {
   for (int64_t i = 0; i < n; ++i) mx = std::max<int64_t>(mx, htp[i + 1] - htp[i]);
   m->max_trow = mx;
   if (nlong > 0) {
-    KS_CHECK(mx <= kSortMax, KS_ERR_UNSUPPORTED,
-             "transpose: a column holds " + std::to_string(mx) + " entries; at most " +
-                 std::to_string(kSortMax) + " are supported");
     k_sort_long<<<nlong, 1024, 0, s>>>(m->tptr.p, longrows.p, m->tcol.p, m->tperm.p);
     KS_HIPCK(hipGetLastError());
   }
+  if (mx > kSortMax) sort_huge_segments(m, htp, s);
   if (nnz > 0) {
     k_gather<double, double><<<grid_map(nnz), kBlock, 0, s>>>(nnz, m->tperm.p, m->val.p, m->tval.p);
     KS_HIPCK(hipGetLastError());

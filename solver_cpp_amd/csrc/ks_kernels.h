@@ -1162,7 +1162,7 @@ __global__ void k_scan_apply(int64_t n, const int32_t* __restrict__ in, const in
 }
 
 // Scatter entries of A into column buckets of A^T (order inside a bucket is
-// fixed afterwards by k_sort_segments).
+// fixed afterwards by k_sort_small / k_sort_long).
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_tscatter(int64_t n, const int32_t* __restrict__ ptr,
                                                      const int32_t* __restrict__ col,
@@ -1180,8 +1180,12 @@ __global__ __launch_bounds__(kBlock) void k_tscatter(int64_t n, const int32_t* _
   }
 }
 
-// Sort each A^T row segment by column (= row of A), carrying the value
-// permutation: one thread per segment, insertion sort (segments <= 64).
+// Order inside an A^T row segment.  The key is the source entry index
+// (tperm): entries of one column of A taken in CSR order come in row order,
+// ties between repeated (row, column) entries by their position in the row,
+// so the result is the stable column-major order of a counting-sort
+// transpose, duplicates included.  One thread per segment of <= 64 entries
+// (insertion sort); longer segments are listed for k_sort_long.
 __global__ void k_sort_small(int64_t n, const int32_t* __restrict__ tptr, int32_t* tcol, int32_t* tperm,
                              int32_t* long_rows, int32_t* n_long) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1195,7 +1199,7 @@ __global__ void k_sort_small(int64_t n, const int32_t* __restrict__ tptr, int32_
       const int32_t kc = tcol[a];
       const int32_t kp = tperm[a];
       int b = a - 1;
-      while (b >= s && tcol[b] > kc) {
+      while (b >= s && tperm[b] > kp) {
         tcol[b + 1] = tcol[b];
         tperm[b + 1] = tperm[b];
         --b;
@@ -1206,20 +1210,17 @@ __global__ void k_sort_small(int64_t n, const int32_t* __restrict__ tptr, int32_
   }
 }
 
-// One workgroup per long segment (<= kSortMax entries): bitonic sort in LDS.
+// Bitonic sort by tperm of [s, s + len), len <= kSortMax, in LDS by one
+// workgroup of 1024 threads.
 constexpr int kSortMax = 8192;
-__global__ __launch_bounds__(1024) void k_sort_long(const int32_t* __restrict__ tptr,
-                                                    const int32_t* __restrict__ long_rows, int32_t* tcol,
-                                                    int32_t* tperm) {
+__device__ __forceinline__ void bitonic_segment(int64_t s, int len, int32_t* tcol, int32_t* tperm) {
   __shared__ int32_t kc[kSortMax];
   __shared__ int32_t kp[kSortMax];
-  const int32_t r = long_rows[blockIdx.x];
-  const int s = tptr[r], e = tptr[r + 1], len = e - s;
   int P = 1;
   while (P < len) P <<= 1;
   for (int i = threadIdx.x; i < P; i += blockDim.x) {
-    kc[i] = i < len ? tcol[s + i] : 0x7fffffff;
-    kp[i] = i < len ? tperm[s + i] : 0;
+    kp[i] = i < len ? tperm[s + i] : 0x7fffffff;
+    kc[i] = i < len ? tcol[s + i] : 0;
   }
   __syncthreads();
   for (int size = 2; size <= P; size <<= 1) {
@@ -1228,7 +1229,7 @@ __global__ __launch_bounds__(1024) void k_sort_long(const int32_t* __restrict__ 
         const int j = i ^ stride;
         if (j > i) {
           const bool up = (i & size) == 0;
-          if ((kc[i] > kc[j]) == up) {
+          if ((kp[i] > kp[j]) == up) {
             int32_t a = kc[i]; kc[i] = kc[j]; kc[j] = a;
             int32_t b = kp[i]; kp[i] = kp[j]; kp[j] = b;
           }
@@ -1240,6 +1241,58 @@ __global__ __launch_bounds__(1024) void k_sort_long(const int32_t* __restrict__ 
   for (int i = threadIdx.x; i < len; i += blockDim.x) {
     tcol[s + i] = kc[i];
     tperm[s + i] = kp[i];
+  }
+}
+
+// One workgroup per listed segment of 65..kSortMax entries; longer ones are
+// left to k_sort_chunks + k_merge_runs.
+__global__ __launch_bounds__(1024) void k_sort_long(const int32_t* __restrict__ tptr,
+                                                    const int32_t* __restrict__ long_rows, int32_t* tcol,
+                                                    int32_t* tperm) {
+  const int32_t r = long_rows[blockIdx.x];
+  const int s = tptr[r], len = tptr[r + 1] - s;
+  if (len > kSortMax) return;
+  bitonic_segment(s, len, tcol, tperm);
+}
+
+// One workgroup per chunk [cs[b], cs[b] + cl[b]) of a segment longer than
+// kSortMax; chunks are aligned to kSortMax from the segment start.
+__global__ __launch_bounds__(1024) void k_sort_chunks(const int32_t* __restrict__ cs, const int32_t* __restrict__ cl,
+                                                      int32_t* tcol, int32_t* tperm) {
+  bitonic_segment(cs[blockIdx.x], cl[blockIdx.x], tcol, tperm);
+}
+
+// One merge pass over the long segments: segment h holds len_h = off[h+1] -
+// off[h] entries, at src + sb[h] (sb = the segment start in A^T, or off[h] in
+// the flat staging buffer) and goes to dst + db[h].  Sorted runs of width w
+// are merged pairwise: an entry's place is its index in its own run plus the
+// number of keys in the partner run below its own (keys are distinct).
+__global__ void k_merge_runs(int64_t total, int32_t nh, const int64_t* __restrict__ off,
+                             const int64_t* __restrict__ sb, const int64_t* __restrict__ db, int64_t w,
+                             const int32_t* __restrict__ sc, const int32_t* __restrict__ sp,
+                             int32_t* __restrict__ dc, int32_t* __restrict__ dp) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    int lo = 0, hi = nh - 1;  // last h with off[h] <= e
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    const int64_t i = e - off[lo], len = off[lo + 1] - off[lo];
+    const int64_t run = i / w, pos = i - run * w, partner = run ^ 1;
+    const int64_t pb = partner * w, plen = pb < len ? (len - pb < w ? len - pb : w) : 0;
+    const int64_t s0 = sb[lo];
+    const int32_t key = sp[s0 + i];
+    int64_t a = 0, b = plen;  // count of partner keys < key
+    while (a < b) {
+      const int64_t mid = (a + b) >> 1;
+      if (sp[s0 + pb + mid] < key) a = mid + 1;
+      else b = mid;
+    }
+    const int64_t o = (run < partner ? run : partner) * w + pos + a;
+    dp[db[lo] + o] = key;
+    dc[db[lo] + o] = sc[s0 + i];
   }
 }
 

@@ -135,6 +135,48 @@ def test_spmv_long_rows(ctx, oracle, layout):
     assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
 
 
+def _dense_column_csr(n, seed):
+    # a diagonal plus three dense columns of A (rows of A^T longer than one
+    # LDS sort of 8192 entries: 1, 2 and 4 merge passes), unsorted column
+    # order inside rows and repeated (row, column) entries
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        r = [i, 0]
+        if i % 5 == 0:
+            r.append(1)
+        if i % 12 == 0:
+            r.append(2)
+        if i % 997 == 3:
+            r += [3, 3]
+        rng.shuffle(r)
+        rows.append(r)
+    ptr = np.zeros(n + 1, np.int32)
+    ptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.fromiter((c for r in rows for c in r), np.int32, count=int(ptr[-1]))
+    val = rng.standard_normal(len(col))
+    return ptr, col, val
+
+
+@pytest.mark.parametrize("layout", ["auto", "stream"])
+def test_transpose_dense_columns(ctx, oracle, layout):
+    ptr, col, val = _dense_column_csr(100_000, 17)
+    n = len(ptr) - 1
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
+    tp, tc, tv = A.transpose_csr()
+    otp, otc, otv = oracle.transpose(ptr, col, val)
+    assert np.diff(otp)[:3].tolist() == [n + 1, 20_001, 8_335]  # the diagonal adds one each
+    assert np.array_equal(tp, otp) and np.array_equal(tc, otc) and np.array_equal(tv, otv)
+    x = np.random.default_rng(5).standard_normal(n)
+    yt = A.spmv(x, trans=True)
+    yto = oracle.spmv(otp, otc, otv, x)
+    assert np.all(np.abs(yt - yto) <= 4 * np.diff(otp).max() * EPS64 * P.row_abs_bound(otp, otc, otv, x) + 1e-300)
+    # refreshed values follow the same permutation
+    A.update_values(3.0 * val)
+    assert np.array_equal(A.transpose_csr()[2], 3.0 * otv)
+    A.close()
+
+
 def test_borrowed_matrix(ctx, oracle):
     # KS_MAT_BORROW: the caller's device CSR is used in place
     import torch

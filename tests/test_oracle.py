@@ -95,6 +95,23 @@ def test_transpose(oracle):
     assert abs(y @ oracle.spmv(ptr, col, val, x) - x @ oracle.spmv(tp, tc, tv, y)) < 1e-10
 
 
+def test_transpose_repeated_entries(oracle):
+    # unsorted rows with repeated (row, column) entries: A^T keeps the entries
+    # of each column in CSR order, i.e. a stable sort of the entries by column
+    rng = np.random.default_rng(4)
+    n = 300
+    rl = rng.integers(0, 9, n)
+    ptr = np.zeros(n + 1, np.int32)
+    ptr[1:] = np.cumsum(rl)
+    col = rng.integers(0, 40, int(ptr[-1])).astype(np.int32)  # many repeats
+    val = rng.standard_normal(len(col))
+    tp, tc, tv = oracle.transpose(ptr, col, val)
+    order = np.argsort(col, kind="stable")
+    rows = np.repeat(np.arange(n, dtype=np.int32), rl)
+    assert np.array_equal(tp, np.concatenate([[0], np.cumsum(np.bincount(col, minlength=n))]))
+    assert np.array_equal(tc, rows[order]) and np.array_equal(tv, val[order])
+
+
 @pytest.mark.parametrize("n", [1, 10, 12345, 1_000_003])
 def test_dot_norm(oracle, n):
     i = np.arange(n, dtype=np.float64)
