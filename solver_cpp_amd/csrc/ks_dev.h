@@ -503,14 +503,25 @@ struct EpiStore {
   __device__ void row(int64_t i, T s, T, double*) { y[i] = s; }
   __device__ void fin(const double*) const {}
 };
-// y = (A x) * (1 / *d) (d: device FP64 scalar; the reciprocal is formed
-// once per thread, in T)
+// y = (A x) * r (r: device FP64 reciprocal of the basis norm, formed by the
+// norm finisher; rounded to T once per thread and held in scalar registers,
+// as it is uniform)
 template <class T>
 struct EpiDiv {
   T* y;
-  const double* d;
+  const double* r;
   T inv;
-  __device__ void prep() { inv = (T)(1.0 / *d); }
+  __device__ void prep() {
+    const T v = (T)*r;
+    if constexpr (sizeof(T) == 4) {
+      inv = __builtin_bit_cast(T, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+    } else {
+      const uint64_t b = __builtin_bit_cast(uint64_t, v);
+      const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+      const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+      inv = __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    }
+  }
   __device__ void row(int64_t i, T s, T, double*) { y[i] = s * inv; }
   __device__ void fin(const double*) const {}
 };

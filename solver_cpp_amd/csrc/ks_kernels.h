@@ -50,6 +50,7 @@ struct Ctl {
   int R, ld;       // GMRES restart and H row stride
   double *H, *e, *cs, *sn, *y;
   double* vn;      // GMRES: norms of the unnormalised basis vectors (R)
+  double* vr;      // GMRES: 1 / vn (R), the SpMV epilogue's scale
   int kskip, fix, compat, pad;
   double *delta, *eta, *zeta, *rho, *phi;
 };
@@ -269,13 +270,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
 // indices of the workgroup's next tile are already in flight (eight
 // independent loads per thread).  NX = 2 forms A x0 and A x1 from one pass
 // over the matrix (the k-skip matrix-powers basis).
-// Values and column indices are read as pairs (entries e, e+1 with e even:
-// 8-byte column loads, 8- or 16-byte value loads).  A tile's pairs cover the
-// even-aligned window [j0 & ~1, (j0 & ~1) + window), so a tile holds at most
-// window - 1 entries; the matrix's last entry, when it starts a pair, is read
-// alone, so no load passes nnz.
+// Values and column indices are read as quads (entries e..e+3 with e a
+// multiple of 4: 16-byte column loads, 16- or 2x16-byte value loads) and
+// staged in LDS at their offsets from the quad-aligned tile start a0 =
+// j0 & ~3 with aligned 16-byte LDS stores.  A tile's quads cover
+// [a0, a0 + window), so a tile holds at most window - 3 entries; a quad
+// that would pass nnz is read entry by entry, so no load passes nnz.
 constexpr int kTileWin = 2048;
-constexpr int kTileNnz = kTileWin - 1;
+constexpr int kTileNnz = kTileWin - 3;
 constexpr int kTileRows = 256;
 
 template <class T>
@@ -293,60 +295,94 @@ template <class T>
 struct Vec2 {
   typedef T type __attribute__((ext_vector_type(2)));
 };
+template <class T>
+struct Vec4 {
+  typedef T type __attribute__((ext_vector_type(4)));
+};
 
-// One thread's share of a tile: NTH threads (a workgroup or a wavefront)
-// cover a window of WIN entries, thread t taking the pairs at 2t + 2 NTH q.
-template <class T, int NTH, int WIN>
-struct PairTile {
-  static constexpr int kPairs = WIN / (2 * NTH);
-  T v[2 * kPairs];
-  int c[2 * kPairs];
-  int32_t pr;  // row pointer of row r0 + t
-  __device__ __forceinline__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                                       const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1,
-                                       int32_t nnz, int t) {
-    if (j1 - j0 > WIN - 1) return;  // long row: loaded in place
-    const int32_t a0 = j0 & ~1;
+// A chunk of W consecutive entries, loaded and stored as one vector.
+template <class U, int W>
+struct Chunk {
+  typedef U type __attribute__((ext_vector_type(W)));
+};
+
+// One array's share of a tile for one thread: NTH threads cover a window of
+// WIN entries from a0, thread t taking the W-entry chunks at a0 + W t +
+// W NTH i, so each load instruction reads one chunk per lane from
+// consecutive lanes.
+template <class U, int NTH, int WIN, int W>
+struct TileArr {
+  static constexpr int kN = WIN / (W * NTH);
+  U x[kN][W];
+  // kFull: the whole window [a0, a0 + WIN) lies below nnz (uniform across
+  // the tile), so no lane needs the entry-by-entry path; keeping that path
+  // out of the common case keeps the chunk loads back to back.
+  template <bool kFull>
+  __device__ __forceinline__ void load(const U* __restrict__ p, int32_t a0, int32_t j1, int32_t nnz, int t) {
 #pragma unroll
-    for (int q = 0; q < kPairs; ++q) {
-      const int32_t e = a0 + 2 * t + 2 * NTH * q;
+    for (int i = 0; i < kN; ++i) {
+      const int32_t e = a0 + W * t + W * NTH * i;
       if (e < j1) {
-        if (e + 1 < nnz) {
-          const typename Vec2<T>::type vv = ld_stream(reinterpret_cast<const typename Vec2<T>::type*>(val + e));
-          const typename Vec2<int>::type cc = ld_stream(reinterpret_cast<const typename Vec2<int>::type*>(col + e));
-          v[2 * q] = vv[0];
-          v[2 * q + 1] = vv[1];
-          c[2 * q] = cc[0];
-          c[2 * q + 1] = cc[1];
+        if (kFull || e + W - 1 < nnz) {
+          const typename Chunk<U, W>::type q = ld_stream(reinterpret_cast<const typename Chunk<U, W>::type*>(p + e));
+#pragma unroll
+          for (int h = 0; h < W; ++h) x[i][h] = q[h];
         } else {
-          // both halves written on both paths, so the array stays in registers
-          v[2 * q] = ld_stream(val + e);
-          c[2 * q] = ld_stream(col + e);
-          v[2 * q + 1] = 0;
-          c[2 * q + 1] = 0;
+          // every slot written on both paths, so the array stays in registers
+#pragma unroll
+          for (int h = 0; h < W; ++h) x[i][h] = e + h < nnz ? ld_stream(p + e + h) : U(0);
         }
       }
     }
-    pr = t < r1 - r0 ? ptr[r0 + t] : 0;
   }
-  // the tile's entries into sv / sc at their offsets from j0
-  __device__ __forceinline__ void stage(T* sv, int32_t* sc, int32_t j0, int32_t j1, int t) const {
-    const int32_t a0 = j0 & ~1;
+  // chunks at their offsets from a0 (16-byte aligned LDS stores); slots
+  // outside [j0, j1) are written too and never read
+  __device__ __forceinline__ void stage(U* s, int32_t span, int t) const {
 #pragma unroll
-    for (int q = 0; q < kPairs; ++q) {
+    for (int i = 0; i < kN; ++i) {
+      const int32_t k = W * t + W * NTH * i;
+      if (k < span) {
+        typename Chunk<U, W>::type q;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int32_t k = a0 + 2 * t + 2 * NTH * q + h - j0;
-        if (k >= 0 && k < j1 - j0) {
-          sv[k] = v[2 * q + h];
-          sc[k] = c[2 * q + h];
-        }
+        for (int h = 0; h < W; ++h) q[h] = x[i][h];
+        *reinterpret_cast<typename Chunk<U, W>::type*>(s + k) = q;
       }
     }
   }
 };
+
+// One thread's share of a tile: values and column indices of the window
+// [a0, a0 + WIN), a0 = j0 & ~3, plus the row pointer of row r0 + t.  Chunks
+// are 16 bytes of values (4 FP32 or 2 FP64 entries) and as many column
+// indices (16 or 8 bytes).
+template <class T, int NTH, int WIN>
+struct QuadTile {
+  static constexpr int W = 16 / (int)sizeof(T);
+  TileArr<T, NTH, WIN, W> v;
+  TileArr<int32_t, NTH, WIN, W> c;
+  int32_t pr;
+  __device__ __forceinline__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                       const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1,
+                                       int32_t nnz, int t) {
+    if (j1 - j0 > WIN - 3) return;  // long row: loaded in place
+    const int32_t a0 = j0 & ~3;
+    if (a0 + WIN <= nnz) {
+      v.template load<true>(val, a0, j1, nnz, t);
+      c.template load<true>(col, a0, j1, nnz, t);
+    } else {
+      v.template load<false>(val, a0, j1, nnz, t);
+      c.template load<false>(col, a0, j1, nnz, t);
+    }
+    pr = t < r1 - r0 ? ptr[r0 + t] : 0;
+  }
+  __device__ __forceinline__ void stage(T* sv, int32_t* sc, int32_t j0, int32_t j1, int t) const {
+    const int32_t span = j1 - (j0 & ~3);
+    v.stage(sv, span, t);
+    c.stage(sc, span, t);
+  }
+};
 template <class T>
-using TileData = PairTile<T, kBlock, kTileWin>;
+using TileData = QuadTile<T, kBlock, kTileWin>;
 
 // Register budget of the CSR-stream kernels as waves/SIMD: FP32 64 VGPRs
 // (8 waves) for a plain SpMV and 72 (7 waves) with a fused reduction, FP64
@@ -369,8 +405,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
   if (g.off()) return;
   st.count();
   epi.prep();
-  __shared__ T sv[kTileWin];
-  __shared__ int32_t sc[kTileWin];
+  __shared__ __align__(16) T sv[kTileWin];
+  __shared__ __align__(16) int32_t sc[kTileWin];
   __shared__ int32_t sptr[kTileRows + 1];
   __shared__ double slong[NX][kBlock / 64];
   double acc[NV > 0 ? NV : 1];
@@ -430,8 +466,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       __syncthreads();
     } else {
-      if (tid < nr) sptr[tid] = d.pr - j0;
-      if (tid == 0) sptr[nr] = cnt;
+      if (tid < nr) sptr[tid] = d.pr - (j0 & ~3);
+      if (tid == 0) sptr[nr] = j1 - (j0 & ~3);
       d.stage(sv, sc, j0, j1, tid);
       __syncthreads();
       // in flight during the sums: the next tile's data, then the meta after it
@@ -475,7 +511,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
 // moving LDS accesses across them) order its staging writes before its row
 // sums and its sums before the next tile's writes.
 constexpr int kWaveWin = 512;
-constexpr int kWaveNnz = kWaveWin - 1;
+constexpr int kWaveNnz = kWaveWin - 3;
 constexpr int kWaveRows = 64;
 
 __device__ inline void wave_sync() {
@@ -485,7 +521,7 @@ __device__ inline void wave_sync() {
 }
 
 template <class T>
-using WaveTile = PairTile<T, 64, kWaveWin>;
+using WaveTile = QuadTile<T, 64, kWaveWin>;
 
 template <class T, int NX, int NV, class Epi>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamWaves<T, NX, NV>))) void k_spmv_wave(
@@ -496,8 +532,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
   st.count();
   epi.prep();
   constexpr int kW = kBlock / 64;
-  __shared__ T sv_all[kW][kWaveWin];
-  __shared__ int32_t sc_all[kW][kWaveWin];
+  __shared__ __align__(16) T sv_all[kW][kWaveWin];
+  __shared__ __align__(16) int32_t sc_all[kW][kWaveWin];
   __shared__ int32_t sptr_all[kW][kWaveRows + 1];
   double acc[NV > 0 ? NV : 1];
 #pragma unroll
@@ -543,8 +579,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
       }
       if (lane == 0) epi.row(r0, s0, s1, acc);
     } else {
-      if (lane < nr) sptr[lane] = d.pr - j0;
-      if (lane == 0) sptr[nr] = cnt;
+      if (lane < nr) sptr[lane] = d.pr - (j0 & ~3);
+      if (lane == 0) sptr[nr] = j1 - (j0 & ~3);
       d.stage(sv, sc, j0, j1, lane);
       wave_sync();
       // in flight during the sums: the next tile's data, then the bounds after it
@@ -975,6 +1011,7 @@ __device__ inline void gm_norm_step(Ctl* c, int k, double h) {
   double* __restrict__ sn = c->sn;
   double* __restrict__ e = c->e;
   double* __restrict__ vn = c->vn;
+  double* __restrict__ vr = c->vr;
   const int ld = c->ld, R = c->R;
   const long long count = c->count, steps = c->steps, tsteps = c->tsteps, maxloop = c->maxloop;
   const long long nhist = c->nhist, hist_cap = c->hist_cap;
@@ -1013,6 +1050,7 @@ __device__ inline void gm_norm_step(Ctl* c, int k, double h) {
   H[k * ld + k] = ck * hk - sk * h;
   H[(k + 1) * ld + k] = 0.0;
   vn[k + 1] = h;
+  vr[k + 1] = 1.0 / h;
   c->vscale = h;
   c->count = count + 1;
   c->steps = steps + 1;

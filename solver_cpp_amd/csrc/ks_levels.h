@@ -264,6 +264,7 @@ struct FinGmCycle {
     c->e[0] = t;
     c->vscale = t;
     c->vn[0] = t;
+    c->vr[0] = 1.0 / t;
     gm_check(c, 0, t);
   }
 };
@@ -404,7 +405,7 @@ struct Level {
     hctl.fix = cfg.fix;
     hctl.compat = cfg.compat;
     const int64_t nH = R > 0 ? (int64_t)(R + 1) * R : 0;
-    const int64_t nsmall = R > 0 ? 5 * (int64_t)R : 0;
+    const int64_t nsmall = R > 0 ? 6 * (int64_t)R : 0;
     const int64_t nk = 5 * (int64_t)(2 * k + 3);
     arr.alloc((size_t)(nH + nsmall + nk + 8));
     KS_HIPCK(hipMemsetAsync(arr.p, 0, arr.n * sizeof(double), c->stream));
@@ -416,6 +417,7 @@ struct Level {
       hctl.sn = q; q += R;
       hctl.y = q; q += R;
       hctl.vn = q; q += R;
+      hctl.vr = q; q += R;
     }
     hctl.delta = q; q += 2 * k + 3;
     hctl.eta = q; q += 2 * k + 3;
@@ -769,7 +771,7 @@ struct GmresLevel : Level {
       launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
       launch_map<0>(c, n, OpScale<T>{w.p, vk1, h, 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
     } else {
-      launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w.p, hctl.vn + k}, gstep());
+      launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w.p, hctl.vr + k}, gstep());
       launch_proj<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, gstep(), hctl.vn);
       launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, vk1, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep(),
                            hctl.vn);
