@@ -525,6 +525,15 @@ struct EpiDiv {
   __device__ void row(int64_t i, T s, T, double*) { y[i] = s * inv; }
   __device__ void fin(const double*) const {}
 };
+// EpiDiv plus the projection h_v = (y, V_v), v < nv (nv <= kMaxNV), formed
+// by the workgroup CSR-stream kernel over its own rows after the SpMV.
+template <class T>
+struct EpiDivProj : EpiDiv<T> {
+  const T* V;
+  int64_t ldv;
+  int nv;
+  static constexpr bool kProj = true;
+};
 template <class T>
 struct EpiStore2 {
   T *y0, *y1;

@@ -22,6 +22,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace ks {
 
@@ -129,29 +130,49 @@ __device__ void block_store(double (&acc)[NV], RedArgs ra) {
 
 // Totals of the G partials of each of the NV values, then f.fin(sums) on one
 // thread.  TPV = kBlock / NVP threads share one value (NVP = NV rounded up to
-// a power of two, at least 4): each sums a fixed strided subset with all
-// loads independent, then a shuffle tree inside the TPV-lane group - a fixed
-// order, so the totals are deterministic.
+// a power of two, at least 4): each sums a fixed strided subset in rounds of
+// sixteen (then four, then one) independent loads (the partials come from
+// other XCDs' writes, so each round is a full memory round trip: few rounds
+// keep the finisher short), then a fixed tree over the sixteen sums and a
+// shuffle tree inside the TPV-lane group - a fixed order, so the totals are
+// deterministic.
 template <int NV, class F>
 __global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restrict__ part, F f, Gate g) {
   if (g.off()) return;
   constexpr int NVP = NV <= 4 ? 4 : NV <= 8 ? 8 : NV <= 16 ? 16 : NV <= 32 ? 32 : 64;
   constexpr int TPV = kBlock / NVP;  // 4..64, a divisor of the wavefront
+  constexpr int U = 16;
   __shared__ double sums[NVP];
   const int v = threadIdx.x / TPV, j = threadIdx.x % TPV;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  double a[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) a[u] = 0.0;
   if (v < NV) {
     const double* p = part + (size_t)v * kMaxGrid;
     int b = j;
-    for (; b + 3 * TPV < G; b += 4 * TPV) {
-      s0 += p[b];
-      s1 += p[b + TPV];
-      s2 += p[b + 2 * TPV];
-      s3 += p[b + 3 * TPV];
+    // full rounds of U loads, then of 4, then single loads; every load in a
+    // round is unconditional, so the round's loads issue back to back
+    for (; b + (U - 1) * TPV < G; b += U * TPV) {
+      double t[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) t[u] = p[b + u * TPV];
+#pragma unroll
+      for (int u = 0; u < U; ++u) a[u] += t[u];
     }
-    for (; b < G; b += TPV) s0 += p[b];
+    for (; b + 3 * TPV < G; b += 4 * TPV) {
+      double t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = p[b + u * TPV];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] += t[u];
+    }
+    for (; b < G; b += TPV) a[0] += p[b];
   }
-  double s = (s0 + s1) + (s2 + s3);
+#pragma unroll
+  for (int w = U / 2; w > 0; w >>= 1)
+#pragma unroll
+    for (int u = 0; u < w; ++u) a[u] += a[u + w];
+  double s = a[0];
 #pragma unroll
   for (int off = TPV / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (j == 0) sums[v] = s;
@@ -279,6 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
 constexpr int kTileWin = 2048;
 constexpr int kTileNnz = kTileWin - 3;
 constexpr int kTileRows = 256;
+static_assert(kTileRows <= kBlock, "a workgroup tile row per thread");
 
 template <class T>
 __device__ inline T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
@@ -394,6 +416,14 @@ constexpr int kStreamWaves = sizeof(T) == 4 ? (NV == 0 ? 8 : 7) : (NX == 2 ? 4 :
 template <class T, int NX, int NV>
 constexpr int kWgWaves = sizeof(T) == 4 ? (NV == 0 ? 7 : 6) : kStreamWaves<T, NX, NV>;
 
+// An epilogue with `kProj` also asks the workgroup kernel for the partial
+// projections (y, V_v), v < nv, over the rows it produced (see
+// k_spmv_stream).
+template <class E, class = void>
+struct HasProj : std::false_type {};
+template <class E>
+struct HasProj<E, std::void_t<decltype(E::kProj)>> : std::true_type {};
+
 template <class T, int NX, int NV, class Epi>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves<T, NX, NV>))) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
                                                         const int32_t* __restrict__ tnz,
@@ -499,6 +529,62 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
     nr0 = mr0; nr1 = mr1; nj0 = mj0; nj1 = mj1;
   }
   if constexpr (NV > 0) block_store<NV>(acc, ra);
+  if constexpr (HasProj<Epi>::value) {
+    // Classical Gram-Schmidt projection fused behind the SpMV: the partial
+    // dots (y, V_v) over this workgroup's own rows, which it has just
+    // written (visible to the whole workgroup after the loop's last
+    // barrier; still in this XCD's L2), so y is not read back from HBM and
+    // no separate projection launch runs.  Each wavefront takes every
+    // fourth of the workgroup's tiles and reads 16-byte chunks of rows
+    // (y, V and ldv 16-byte aligned, checked by the host); a tile's rows
+    // outside its aligned chunks go one per lane.  Eight columns per pass,
+    // each pass's partials to part[v * kMaxGrid + workgroup].
+    static_assert(NV == 0 && NX == 1, "projection epilogue takes a plain SpMV");
+    using VT = typename Chunk<T, 16 / sizeof(T)>::type;
+    constexpr int W = 16 / sizeof(T);
+    constexpr int kW = kBlock / 64;
+    const int lane = tid & 63, wv = tid >> 6;
+    int64_t u0, uend, ustep;
+    rbs.range(u0, uend, ustep);
+    const int nv = epi.nv;
+    const int64_t ldv = epi.ldv;
+    for (int v0 = 0; v0 < nv; v0 += 8) {
+      double pa[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) pa[j] = 0.0;
+      const T* vb = epi.V + (int64_t)v0 * ldv;
+      for (int64_t u = u0 + (int64_t)wv * ustep; u < uend; u += (int64_t)kW * ustep) {
+        const int64_t q0 = tiles[u], q1 = tiles[u + 1];
+        const int64_t a = (q0 + W - 1) / W * W, b = q1 / W * W;
+        for (int64_t r = a + (int64_t)W * lane; r + W <= b; r += 64 * W) {
+          const VT yv = *reinterpret_cast<const VT*>(epi.y + r);
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (v0 + j < nv) {
+              const VT xv = *reinterpret_cast<const VT*>(vb + (int64_t)j * ldv + r);
+              double t = 0.0;
+#pragma unroll
+              for (int e = 0; e < W; ++e) t += (double)yv[e] * (double)xv[e];
+              pa[j] += t;
+            }
+        }
+        // rows before the first and after the last aligned chunk (< 2W)
+        const int64_t hn = (a < q1 ? a : q1) - q0;
+        const int64_t tb = b > a ? b : a;
+        int64_t row = -1;
+        if (lane < hn) row = q0 + lane;
+        else if (lane >= W && lane - W < q1 - tb) row = tb + (lane - W);
+        if (row >= 0) {
+          const double wr = (double)epi.y[row];
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (v0 + j < nv) pa[j] += wr * (double)vb[(int64_t)j * ldv + row];
+        }
+      }
+      if (v0 > 0) __syncthreads();  // block_store's staging is reused
+      block_store<8>(pa, RedArgs{ra.part + (size_t)v0 * kMaxGrid});
+    }
+  }
 }
 
 
@@ -528,6 +614,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
     int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles, const int32_t* __restrict__ tnz,
     const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x0, const T* __restrict__ x1, Epi epi, Gate g, RedArgs ra, StatArgs st) {
+  static_assert(!HasProj<Epi>::value, "the projection epilogue is a workgroup-tile kernel feature");
   if (g.off()) return;
   st.count();
   epi.prep();

@@ -662,6 +662,33 @@ void launch_proj(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T*
   }
 }
 
+// w = (A vk) * r and h_i = (w, V_i) / vn_i, i < nv.  On workgroup tiles one
+// launch does both (the CSR-stream kernel's projection epilogue), then the
+// projection finisher; otherwise the SpMV and a separate projection.
+template <class T>
+void launch_spmv_proj(ks_ctx* c, const MatView<T>& A, const T* vk, T* w, const double* r, const T* V, int64_t ldv,
+                      int nv, double* hcol, int ldh, Gate g, const double* vn) {
+  if (A.kind == 0 && A.wg_tiles && nv <= kMaxNV && vec_ok(V, ldv, (const T*)w)) {
+    const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
+    const double bytes = spmv_bytes<T>(A.n, A.nnz) + (double)sizeof(T) * A.n * nv;
+    EpiDivProj<T> e{};
+    e.y = w; e.r = r; e.V = V; e.ldv = ldv; e.nv = nv;
+    auto kern = k_spmv_stream<T, 1, 0, EpiDivProj<T>>;
+    const int G = grid_rows(c, kern, A.ntiles);
+    klaunch(c, cls, kern, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, vk, vk, e, g, red(c),
+            stat(c, cls, bytes));
+    const FinProj f{hcol, ldh, nv, vn};
+    if (nv <= 4) launch_finish<4>(c, G, f, g);
+    else if (nv <= 8) launch_finish<8>(c, G, f, g);
+    else if (nv <= 16) launch_finish<16>(c, G, f, g);
+    else if (nv <= 32) launch_finish<32>(c, G, f, g);
+    else launch_finish<64>(c, G, f, g);
+  } else {
+    launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w, r}, g);
+    launch_proj<T>(c, A.n, V, ldv, nv, w, hcol, ldh, g, vn);
+  }
+}
+
 template <class Fin>
 struct FinWrap {
   Fin f;
@@ -771,8 +798,7 @@ struct GmresLevel : Level {
       launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
       launch_map<0>(c, n, OpScale<T>{w.p, vk1, h, 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
     } else {
-      launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w.p, hctl.vr + k}, gstep());
-      launch_proj<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, gstep(), hctl.vn);
+      launch_spmv_proj<T>(c, A, vk, w.p, hctl.vr + k, V.p, ldv, k + 1, hctl.H + k, hctl.ld, gstep(), hctl.vn);
       launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, vk1, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep(),
                            hctl.vn);
     }

@@ -8,7 +8,8 @@
 `traffic` turns one FETCH_SIZE run and one WRITE_SIZE run into HBM bytes per
 launch of one kernel: each counter is scaled by what it reads for the plain
 device copy of B bytes in the same run (the largest copyBuffer dispatch;
-B/2 read and B/2 written), and the kernel's median over its working
+B/2 read and B/2 written), and the kernel's median (or, with --stat mean,
+mean: for a kernel whose work varies from launch to launch) over its working
 dispatches (FETCH_SIZE above 1 MiB, WRITE_SIZE above 64 KiB: gated
 launches that returned at once are left out) is reported, with a final machine-readable line
 `traffic_bytes_per_launch <kernel> <bytes>`.
@@ -69,7 +70,7 @@ def _counter(db, counter):
     return [(short(k), v) for k, c, v in rows if c == counter]
 
 
-def traffic(fetch_db, write_db, kernel, copy_bytes):
+def traffic(fetch_db, write_db, kernel, copy_bytes, stat="median"):
     fr, wr = _counter(fetch_db, "FETCH_SIZE"), _counter(write_db, "WRITE_SIZE")
     cf = max(v for k, v in fr if "copyBuffer" in k)
     cw = max(v for k, v in wr if "copyBuffer" in k)
@@ -77,11 +78,12 @@ def traffic(fetch_db, write_db, kernel, copy_bytes):
     w_factor = (copy_bytes / 2) / (cw * 1024.0)
     kf = [v for k, v in fr if k == kernel and v > 1024]
     kw = [v for k, v in wr if k == kernel and v > 64]
-    mf, mw = statistics.median(kf), statistics.median(kw)
+    fn = statistics.mean if stat == "mean" else statistics.median
+    mf, mw = fn(kf), fn(kw)
     t = mf * 1024.0 * f_factor + mw * 1024.0 * w_factor
     out = [f"# calibration: copy of {copy_bytes} bytes: FETCH_SIZE {cf:.0f} KiB (factor {f_factor:.3f}), "
            f"WRITE_SIZE {cw:.0f} KiB (factor {w_factor:.3f})",
-           f"# {kernel}: {len(kf)} working dispatches, median FETCH_SIZE {mf:.1f} KiB, WRITE_SIZE {mw:.1f} KiB",
+           f"# {kernel}: {len(kf)} working dispatches, {stat} FETCH_SIZE {mf:.1f} KiB, WRITE_SIZE {mw:.1f} KiB",
            f"# HBM bytes per launch = {mf:.1f} KiB x {f_factor:.3f} + {mw:.1f} KiB x {w_factor:.3f} "
            f"= {t / 1e6:.1f} MB",
            f"traffic_bytes_per_launch {kernel.replace(' ', '')} {t:.0f}"]
@@ -97,13 +99,14 @@ def main():
     ap.add_argument("--kernel", default=None)
     ap.add_argument("--write-db", default=None)
     ap.add_argument("--copy-bytes", type=float, default=0)
+    ap.add_argument("--stat", choices=["median", "mean"], default="median")
     a = ap.parse_args()
     if a.mode == "trace":
         print(trace(a.db, a.top, a.active_us, a.kernel))
     elif a.mode == "pmc":
         print(pmc(a.db, a.kernel))
     else:
-        print(traffic(a.db, a.write_db, a.kernel, a.copy_bytes))
+        print(traffic(a.db, a.write_db, a.kernel, a.copy_bytes, a.stat))
 
 
 if __name__ == "__main__":
