@@ -144,6 +144,7 @@ struct ks_ctx {
   bool own_stream = true;
   std::string err;
   ks::DBuf<double> part;                 // kMaxGrid * kMaxNV reduction partials
+  ks::DBuf<double> part2;                // kMaxGrid partials of a norm whose finisher is deferred
   ks::DBuf<unsigned long long> stats;    // launches[8], bytes[8]
   ks::DBuf<double> scal;                 // scalar outputs of the BLAS entry points
   int* hflags = nullptr;                 // host-mapped flags (host view)
@@ -310,8 +311,8 @@ void klaunch(ks_ctx* c, int cls, void (*kernel)(P...), int G, size_t shmem, A&&.
 
 // Finisher of a reduction whose producer ran on G workgroups.
 template <int NV, class F>
-void launch_finish(ks_ctx* c, int G, const F& f, Gate g) {
-  k_finish<NV, F><<<1, kBlock, 0, c->stream>>>(G, c->part.p, f, g);
+void launch_finish(ks_ctx* c, int G, const F& f, Gate g, const double* part = nullptr) {
+  k_finish<NV, F><<<1, kBlock, 0, c->stream>>>(G, part ? part : c->part.p, f, g);
   KS_HIPCK(hipGetLastError());
 }
 
@@ -525,14 +526,47 @@ struct EpiDiv {
   __device__ void row(int64_t i, T s, T, double*) { y[i] = s * inv; }
   __device__ void fin(const double*) const {}
 };
+// sqrt of the sum of the ng <= kMaxGrid partials p[] (a norm whose update
+// kernel deferred its finisher), formed by the whole workgroup in a fixed
+// order: every workgroup, and every kernel, that calls it gets the same
+// bits.  All threads must call it; all return the value.
+__device__ inline double norm_from_partials(const double* __restrict__ p, int ng) {
+  static_assert(kMaxGrid == 8 * kBlock && kBlock == 256, "eight partials per thread, four wavefronts");
+  __shared__ double sw[kBlock / 64];
+  double t[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) t[u] = p[threadIdx.x + u * kBlock];  // in bounds: kMaxGrid entries
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if ((int)threadIdx.x + u * kBlock >= ng) t[u] = 0.0;
+  double q = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+  q = wave_sum(q);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = q;
+  __syncthreads();
+  return sqrt((sw[0] + sw[1]) + (sw[2] + sw[3]));
+}
+
 // EpiDiv plus the projection h_v = (y, V_v), v < nv (nv <= kMaxNV), formed
 // by the workgroup CSR-stream kernel over its own rows after the SpMV.
+// With np set, the scale is taken from the previous GMRES step's deferred
+// norm: 1 / sqrt(sum of the ng partials np[] its update kernel left), formed
+// by every workgroup with norm_from_partials; that step's norm finisher
+// itself runs in this launch's projection finisher (FinNormProj).
 template <class T>
 struct EpiDivProj : EpiDiv<T> {
   const T* V;
   int64_t ldv;
   int nv;
+  const double* np;
+  int ng;
   static constexpr bool kProj = true;
+  __device__ void prep() {
+    if (!np) {
+      EpiDiv<T>::prep();
+      return;
+    }
+    this->inv = (T)(1.0 / norm_from_partials(np, ng));
+  }
 };
 template <class T>
 struct EpiStore2 {

@@ -491,6 +491,8 @@ ks_status ks_ctx_create(int32_t device, ks_ctx** out) {
     KS_HIPCK(hipSetDevice(device));
     KS_HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->part.alloc((size_t)kMaxGrid * kMaxNV);
+    c->part2.alloc((size_t)kMaxGrid);
+    KS_HIPCK(hipMemsetAsync(c->part2.p, 0, sizeof(double) * kMaxGrid, c->stream));
     c->stats.alloc(2 * KS_K_NCLASS);
     c->scal.alloc(1024);
     KS_HIPCK(hipHostMalloc((void**)&c->hflags, 64 * sizeof(int), hipHostMallocMapped));

@@ -136,9 +136,17 @@ __device__ void block_store(double (&acc)[NV], RedArgs ra) {
 // keep the finisher short), then a fixed tree over the sixteen sums and a
 // shuffle tree inside the TPV-lane group - a fixed order, so the totals are
 // deterministic.
+template <class F, class = void>
+struct HasPre : std::false_type {};
+template <class F>
+struct HasPre<F, std::void_t<decltype(std::declval<F&>().pre())>> : std::true_type {};
+
+// F may have pre(), run by every thread before the totals (a workgroup-wide
+// step of its own, such as norm_from_partials).
 template <int NV, class F>
 __global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restrict__ part, F f, Gate g) {
   if (g.off()) return;
+  if constexpr (HasPre<F>::value) f.pre();
   constexpr int NVP = NV <= 4 ? 4 : NV <= 8 ? 8 : NV <= 16 ? 16 : NV <= 32 ? 32 : 64;
   constexpr int TPV = kBlock / NVP;  // 4..64, a divisor of the wavefront
   constexpr int U = 16;
@@ -434,7 +442,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
                                                         StatArgs st) {
   if (g.off()) return;
   st.count();
-  epi.prep();
   __shared__ __align__(16) T sv[kTileWin];
   __shared__ __align__(16) int32_t sc[kTileWin];
   __shared__ int32_t sptr[kTileRows + 1];
@@ -459,6 +466,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
   if (t + step < end) {
     nr0 = tiles[t + step]; nr1 = tiles[t + step + 1]; nj0 = tnz[t + step]; nj1 = tnz[t + step + 1];
   }
+  // the epilogue's scalars, with the first tile already in flight (an
+  // epilogue may reduce partials across the workgroup here)
+  epi.prep();
   for (; t < end; t += step) {
     const int nr = r1 - r0, cnt = j1 - j0;
     const int64_t tn = t + step, tnn = t + 2 * step;

@@ -662,56 +662,106 @@ void launch_proj(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T*
   }
 }
 
-// w = (A vk) * r and h_i = (w, V_i) / vn_i, i < nv.  On workgroup tiles one
-// launch does both (the CSR-stream kernel's projection epilogue), then the
-// projection finisher; otherwise the SpMV and a separate projection.
-template <class T>
-void launch_spmv_proj(ks_ctx* c, const MatView<T>& A, const T* vk, T* w, const double* r, const T* V, int64_t ldv,
-                      int nv, double* hcol, int ldh, Gate g, const double* vn) {
-  if (A.kind == 0 && A.wg_tiles && nv <= kMaxNV && vec_ok(V, ldv, (const T*)w)) {
-    const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
-    const double bytes = spmv_bytes<T>(A.n, A.nnz) + (double)sizeof(T) * A.n * nv;
-    EpiDivProj<T> e{};
-    e.y = w; e.r = r; e.V = V; e.ldv = ldv; e.nv = nv;
-    auto kern = k_spmv_stream<T, 1, 0, EpiDivProj<T>>;
-    const int G = grid_rows(c, kern, A.ntiles);
-    klaunch(c, cls, kern, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, vk, vk, e, g, red(c),
-            stat(c, cls, bytes));
-    const FinProj f{hcol, ldh, nv, vn};
-    if (nv <= 4) launch_finish<4>(c, G, f, g);
-    else if (nv <= 8) launch_finish<8>(c, G, f, g);
-    else if (nv <= 16) launch_finish<16>(c, G, f, g);
-    else if (nv <= 32) launch_finish<32>(c, G, f, g);
-    else launch_finish<64>(c, G, f, g);
-  } else {
-    launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w, r}, g);
-    launch_proj<T>(c, A.n, V, ldv, nv, w, hcol, ldh, g, vn);
-  }
-}
-
 template <class Fin>
 struct FinWrap {
   Fin f;
   __device__ void fin(const double* s) const { f(s); }
 };
 
+// The projection finisher, sized for nv columns.
+template <class F>
+void launch_proj_finish(ks_ctx* c, int G, int nv, const F& f, Gate g) {
+  if (nv <= 4) launch_finish<4>(c, G, f, g);
+  else if (nv <= 8) launch_finish<8>(c, G, f, g);
+  else if (nv <= 16) launch_finish<16>(c, G, f, g);
+  else if (nv <= 32) launch_finish<32>(c, G, f, g);
+  else launch_finish<64>(c, G, f, g);
+}
+
+// The deferred norm step of GMRES step pk (its ||v_{pk+1}|| from the ng
+// partials np[], formed as the fused SpMV formed it), then the projection
+// finisher of step pk + 1 unless that norm step stopped the solve.
+struct FinNormProj {
+  FinProj p;
+  Ctl* c;
+  int pk;
+  const double* np;
+  int ng;
+  double h;
+  __device__ void pre() { h = norm_from_partials(np, ng); }
+  __device__ void fin(const double* s) {
+    gm_norm_step(c, pk, h);
+    if (c->stop) return;
+    p.fin(s);
+  }
+};
+
+// Whether launch_spmv_proj fuses (and so can also take a deferred norm).
+template <class T>
+bool spmv_proj_fused(const MatView<T>& A, const T* V, int64_t ldv, int nv, const T* w) {
+  return A.kind == 0 && A.wg_tiles && nv <= kMaxNV && vec_ok(V, ldv, w);
+}
+
+// Deferred norm step of GMRES step pk: its update kernel left ng partials of
+// ||v_{pk+1}||^2 in c->part2 and no finisher ran (pk < 0: none pending).
+struct PendNorm {
+  Ctl* c = nullptr;
+  int pk = -1;
+  int ng = 0;
+};
+
+// w = (A vk) * r and h_i = (w, V_i) / vn_i, i < nv.  On workgroup tiles one
+// launch does both (the CSR-stream kernel's projection epilogue), then the
+// projection finisher; otherwise the SpMV and a separate projection.  With
+// pn pending, the previous step's norm finisher runs first: folded into the
+// fused launch (which then takes r from the partials, not from memory), or
+// as its own launch.
+template <class T>
+void launch_spmv_proj(ks_ctx* c, const MatView<T>& A, const T* vk, T* w, const double* r, const T* V, int64_t ldv,
+                      int nv, double* hcol, int ldh, Gate g, const double* vn, PendNorm pn = PendNorm{}) {
+  if (spmv_proj_fused(A, V, ldv, nv, (const T*)w)) {
+    const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
+    const double bytes = spmv_bytes<T>(A.n, A.nnz) + (double)sizeof(T) * A.n * nv;
+    EpiDivProj<T> e{};
+    e.y = w; e.r = r; e.V = V; e.ldv = ldv; e.nv = nv;
+    if (pn.pk >= 0) {
+      e.np = c->part2.p; e.ng = pn.ng;
+    }
+    auto kern = k_spmv_stream<T, 1, 0, EpiDivProj<T>>;
+    const int G = grid_rows(c, kern, A.ntiles);
+    klaunch(c, cls, kern, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, vk, vk, e, g, red(c),
+            stat(c, cls, bytes));
+    if (pn.pk >= 0) launch_proj_finish(c, G, nv, FinNormProj{FinProj{hcol, ldh, nv, vn}, pn.c, pn.pk,
+                                                             c->part2.p, pn.ng, 0.0}, g);
+    else launch_proj_finish(c, G, nv, FinProj{hcol, ldh, nv, vn}, g);
+  } else {
+    if (pn.pk >= 0) launch_finish<1>(c, pn.ng, FinWrap<FinGmNorm>{FinGmNorm{pn.c, pn.pk}}, g, c->part2.p);
+    launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w, r}, g);
+    launch_proj<T>(c, A.n, V, ldv, nv, w, hcol, ldh, g, vn);
+  }
+}
+
 // out = w - V h (out may be w); vn as in launch_proj.
+// defer: the ||out||^2 partials go to c->part2 and no finisher is launched
+// (the next fused SpMV takes them, see EpiDivProj); returns the grid size.
 template <class T, class Fin>
-void launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T* w, T* out,
-                       const double* hcol, int ldh, Fin f, Gate g, const double* vn = nullptr) {
+int launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, const T* w, T* out,
+                      const double* hcol, int ldh, Fin f, Gate g, const double* vn = nullptr, bool defer = false) {
   const double bytes = (double)sizeof(T) * n * (nv + 2);
   const size_t lds = sizeof(double) * std::max(nv, 1);
+  const RedArgs ra = defer ? RedArgs{c->part2.p} : red(c);
   int G;
   if (vec_ok(V, ldv, w) && vec_ok(V, ldv, (const T*)out)) {
     G = grid_map(c, k_cgs_update<T, true>, (n + Vec16<T>::N - 1) / Vec16<T>::N, lds);
-    klaunch(c, KS_K_CGS, k_cgs_update<T, true>, G, lds, n, V, ldv, nv, w, out, hcol, ldh, vn, g, red(c),
+    klaunch(c, KS_K_CGS, k_cgs_update<T, true>, G, lds, n, V, ldv, nv, w, out, hcol, ldh, vn, g, ra,
             stat(c, KS_K_CGS, bytes));
   } else {
     G = grid_map(c, k_cgs_update<T, false>, n, lds);
-    klaunch(c, KS_K_CGS, k_cgs_update<T, false>, G, lds, n, V, ldv, nv, w, out, hcol, ldh, vn, g, red(c),
+    klaunch(c, KS_K_CGS, k_cgs_update<T, false>, G, lds, n, V, ldv, nv, w, out, hcol, ldh, vn, g, ra,
             stat(c, KS_K_CGS, bytes));
   }
-  launch_finish<1>(c, G, FinWrap<Fin>{f}, g);
+  if (!defer) launch_finish<1>(c, G, FinWrap<Fin>{f}, g);
+  return G;
 }
 
 inline int64_t pad64(int64_t n) { return (n + 63) & ~(int64_t)63; }
@@ -725,6 +775,7 @@ struct GmresLevel : Level {
   int R = 0;
   int64_t ldv = 0;
   DBuf<T> V, w, r, xown;
+  PendNorm pend;  // plain GMRES: a norm step deferred into the next step's SpMV
   DBuf<double> Z;  // VP outer only (T = double)
   T* x = nullptr;
   double* xout = nullptr;
@@ -762,6 +813,7 @@ struct GmresLevel : Level {
     b = bb;
     xout = x_out;
     x = sizeof(T) == 8 ? (T*)x_out : xown.p;
+    pend = PendNorm{};
     if (x0given) launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
   }
   // Plain GMRES keeps its basis unnormalised: V_k holds w_k with norm
@@ -798,9 +850,15 @@ struct GmresLevel : Level {
       launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
       launch_map<0>(c, n, OpScale<T>{w.p, vk1, h, 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
     } else {
-      launch_spmv_proj<T>(c, A, vk, w.p, hctl.vr + k, V.p, ldv, k + 1, hctl.H + k, hctl.ld, gstep(), hctl.vn);
-      launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, vk1, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep(),
-                           hctl.vn);
+      launch_spmv_proj<T>(c, A, vk, w.p, hctl.vr + k, V.p, ldv, k + 1, hctl.H + k, hctl.ld, gstep(), hctl.vn,
+                          pend);
+      pend = PendNorm{};
+      // the norm step moves into the next step's fused SpMV when that step
+      // is in this cycle and fuses (flushed by enqueue_finish otherwise)
+      const bool defer = k < R - 2 && spmv_proj_fused(A, (const T*)V.p, ldv, k + 2, (const T*)w.p);
+      const int G = launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, vk1, hctl.H + k, hctl.ld, FinGmNorm{h, k},
+                                         gstep(), hctl.vn, defer);
+      if (defer) pend = PendNorm{h, k, G};
     }
     if (k == R - 2) cycle_end();
   }
@@ -821,6 +879,10 @@ struct GmresLevel : Level {
     KS_HIPCK(hipGetLastError());
   }
   void enqueue_finish() override {
+    if (pend.pk >= 0) {  // the last enqueued step's norm, when no later step took it
+      launch_finish<1>(c, pend.ng, FinWrap<FinGmNorm>{FinGmNorm{pend.c, pend.pk}}, gstep(), c->part2.p);
+      pend = PendNorm{};
+    }
     cycle_end();  // no-op unless the solve stopped in a cycle not yet assembled
     if (sizeof(T) != 8) {
       k_convert<T, double><<<grid_map(n), kBlock, 0, c->stream>>>(n, x, xout, gouter());
