@@ -551,7 +551,8 @@ __device__ inline double norm_from_partials(const double* __restrict__ p, int ng
 // With np set, the scale is taken from the previous GMRES step's deferred
 // norm: 1 / sqrt(sum of the ng partials np[] its update kernel left), formed
 // by every workgroup with norm_from_partials; that step's norm finisher
-// itself runs in this launch's projection finisher (FinNormProj).
+// itself runs in this launch's projection finisher (FinNormProj).  With
+// neither np nor r set the rows are not scaled.
 template <class T>
 struct EpiDivProj : EpiDiv<T> {
   const T* V;
@@ -561,11 +562,9 @@ struct EpiDivProj : EpiDiv<T> {
   int ng;
   static constexpr bool kProj = true;
   __device__ void prep() {
-    if (!np) {
-      EpiDiv<T>::prep();
-      return;
-    }
-    this->inv = (T)(1.0 / norm_from_partials(np, ng));
+    if (np) this->inv = (T)(1.0 / norm_from_partials(np, ng));
+    else if (this->r) EpiDiv<T>::prep();
+    else this->inv = T(1);  // no scale: y = A x exactly
   }
 };
 template <class T>

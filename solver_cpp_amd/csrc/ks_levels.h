@@ -710,7 +710,8 @@ struct PendNorm {
   int ng = 0;
 };
 
-// w = (A vk) * r and h_i = (w, V_i) / vn_i, i < nv.  On workgroup tiles one
+// w = (A vk) * r (r null: w = A vk) and h_i = (w, V_i) / vn_i (vn null: no
+// division), i < nv.  On workgroup tiles one
 // launch does both (the CSR-stream kernel's projection epilogue), then the
 // projection finisher; otherwise the SpMV and a separate projection.  With
 // pn pending, the previous step's norm finisher runs first: folded into the
@@ -736,7 +737,8 @@ void launch_spmv_proj(ks_ctx* c, const MatView<T>& A, const T* vk, T* w, const d
     else launch_proj_finish(c, G, nv, FinProj{hcol, ldh, nv, vn}, g);
   } else {
     if (pn.pk >= 0) launch_finish<1>(c, pn.ng, FinWrap<FinGmNorm>{FinGmNorm{pn.c, pn.pk}}, g, c->part2.p);
-    launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w, r}, g);
+    if (r) launch_spmv<T, 0>(c, A, vk, EpiDiv<T>{w, r}, g);
+    else launch_spmv<T, 0>(c, A, vk, EpiStore<T>{w}, g);
     launch_proj<T>(c, A.n, V, ldv, nv, w, hcol, ldh, g, vn);
   }
 }
@@ -845,8 +847,7 @@ struct GmresLevel : Level {
       inner->enqueue_init((const double*)vk, zk);
       for (int64_t iu = 0; iu < inner->max_units(); ++iu) inner->enqueue_unit(iu);
       inner->enqueue_finish();
-      launch_spmv<T, 0>(c, A, (const T*)zk, EpiStore<T>{w.p}, gstep());
-      launch_proj<T>(c, n, V.p, ldv, k + 1, w.p, hctl.H + k, hctl.ld, gstep());
+      launch_spmv_proj<T>(c, A, (const T*)zk, w.p, nullptr, V.p, ldv, k + 1, hctl.H + k, hctl.ld, gstep(), nullptr);
       launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
       launch_map<0>(c, n, OpScale<T>{w.p, vk1, h, 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
     } else {
