@@ -514,8 +514,8 @@ struct CgLevel : Level {
     } else {
       T* pprev = (u & 1) ? p.p : p2.p;
       const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
-      // SpMV bytes + r and p_prev read, p written, once each
-      const double bytes = spmv_bytes<T>(A.n, A.nnz) + 3.0 * sizeof(T) * n;
+      // SpMV bytes (p_prev gathered as x, q written) + r read and p written
+      const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * n;
       EpiCgDir<T> e{r.p, pprev, pu, q.p, ctl(), FinAlpha{ctl()}, 0.0};
       auto kern = k_spmv_stream<T, 1, 1, EpiCgDir<T>>;
       const int G = grid_rows(c, kern, A.ntiles);
