@@ -12,7 +12,11 @@
 //    per wavefront) that stage values and column indices in LDS and sum one
 //    row per lane; a merge-path variant (k_merge_carry + k_spmv_merge); and
 //    a lanes-per-row kernel (k_spmv).  Work is assigned to XCDs in
-//    contiguous ranges so x gathers hit the XCD's own L2;
+//    contiguous ranges so x gathers hit the XCD's own L2.  On workgroup
+//    tiles an epilogue can also take work that needs no global
+//    synchronisation after the SpMV: the GMRES projection V^T w over the
+//    workgroup's own rows (kProj), and the CG direction update formed where
+//    x is gathered (xdir);
 //  * solver control flow stays on the device: kernels take a Gate and return
 //    at once when the solve (or the enclosing outer step) has stopped, so
 //    the host can enqueue many iterations between synchronisations;
