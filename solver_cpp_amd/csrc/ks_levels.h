@@ -761,12 +761,12 @@ struct PendNorm {
 };
 
 // w = (A vk) * r (r null: w = A vk) and h_i = (w, V_i) / vn_i (vn null: no
-// division), i < nv.  On workgroup tiles one
-// launch does both (the CSR-stream kernel's projection epilogue), then the
-// projection finisher; otherwise the SpMV and a separate projection.  With
-// pn pending, the previous step's norm finisher runs first: folded into the
-// fused launch (which then takes r from the partials, not from memory), or
-// as its own launch.
+// division), i < nv.  On workgroup tiles one launch does both (the
+// CSR-stream kernel's projection epilogue), then the projection finisher;
+// otherwise the SpMV and a separate projection.  With pn pending, the
+// previous step's norm finisher runs first: folded into the fused launch
+// (which then takes its scale from the partials, not from r), or as its own
+// launch.
 template <class T>
 void launch_spmv_proj(ks_ctx* c, const MatView<T>& A, const T* vk, T* w, const double* r, const T* V, int64_t ldv,
                       int nv, double* hcol, int ldh, Gate g, const double* vn, PendNorm pn = PendNorm{}) {
