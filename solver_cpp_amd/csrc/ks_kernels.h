@@ -15,8 +15,7 @@
 //    contiguous ranges so x gathers hit the XCD's own L2.  On workgroup
 //    tiles an epilogue can also take work that needs no global
 //    synchronisation after the SpMV: the GMRES projection V^T w over the
-//    workgroup's own rows (kProj), and the CG direction update formed where
-//    x is gathered (xdir);
+//    workgroup's own rows (kProj);
 //  * solver control flow stays on the device: kernels take a Gate and return
 //    at once when the solve (or the enclosing outer step) has stopped, so
 //    the host can enqueue many iterations between synchronisations;
@@ -436,27 +435,6 @@ struct HasProj : std::false_type {};
 template <class E>
 struct HasProj<E, std::void_t<decltype(E::kProj)>> : std::true_type {};
 
-// An epilogue with `xdir(c)` supplies the gathered x entries itself (the CG
-// direction p = r + beta p_prev formed at the gather, see EpiCgDir) - a
-// workgroup-tile kernel feature like kProj.
-template <class E, class = void>
-struct HasXDir : std::false_type {};
-template <class E>
-struct HasXDir<E, std::void_t<decltype(std::declval<const E&>().xdir(0))>> : std::true_type {};
-// The CG/BiCG direction update p = beta p_prev + r (blas::Scalar_axy,
-// blas.cpp:335-351), formed in FP64 with one fused multiply-add and rounded
-// to T: the same bits wherever it is evaluated.
-template <class T>
-__device__ __forceinline__ T cg_dir(double b, T p, T r) {
-  return (T)__builtin_fma(b, (double)p, (double)r);
-}
-
-template <class Epi, class T>
-__device__ __forceinline__ T xval(const Epi& e, const T* __restrict__ x, int c) {
-  if constexpr (HasXDir<Epi>::value) return e.xdir(c);
-  else return x[c];
-}
-
 template <class T, int NX, int NV, class Epi>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves<T, NX, NV>))) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
                                                         const int32_t* __restrict__ tnz,
@@ -465,7 +443,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
                                                         const T* __restrict__ val, const T* __restrict__ x0,
                                                         const T* __restrict__ x1, Epi epi, Gate g, RedArgs ra,
                                                         StatArgs st) {
-  static_assert(!HasXDir<Epi>::value || NX == 1, "a direction epilogue takes a plain SpMV");
   if (g.off()) return;
   st.count();
   __shared__ __align__(16) T sv[kTileWin];
@@ -506,7 +483,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
       for (int k = tid; k < cnt; k += kBlock) {
         const T a = ld_stream(val + j0 + k);
         const int c = ld_stream(col + j0 + k);
-        s0 += a * xval(epi, x0, c);
+        s0 += a * x0[c];
         if (NX == 2) s1 += a * x1[c];
       }
       const int lane = tid & 63, w = tid >> 6;
@@ -550,7 +527,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
         for (int k = s + lane; k < e; k += tpr) {
           const T v = sv[k];
           const int c = sc[k];
-          a0 = mul_add_rn(a0, v, xval(epi, x0, c));
+          a0 = mul_add_rn(a0, v, x0[c]);
           if (NX == 2) a1 = mul_add_rn(a1, v, x1[c]);
         }
         for (int off = tpr >> 1; off > 0; off >>= 1) {
@@ -650,8 +627,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
     int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles, const int32_t* __restrict__ tnz,
     const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x0, const T* __restrict__ x1, Epi epi, Gate g, RedArgs ra, StatArgs st) {
-  static_assert(!HasProj<Epi>::value && !HasXDir<Epi>::value,
-                "the projection and direction epilogues are workgroup-tile kernel features");
+  static_assert(!HasProj<Epi>::value, "the projection epilogue is a workgroup-tile kernel feature");
   if (g.off()) return;
   st.count();
   epi.prep();

@@ -85,30 +85,6 @@ struct FinAlpha {
   __device__ void operator()(const double* s) const { c->dotv = s[0]; c->alpha = c->rr / s[0]; }
 };
 
-// CG step q = A p with the direction update of the previous step folded in:
-// p = beta p_prev + r is formed where the SpMV gathers it (xdir) and for
-// each own row, written to pout (a buffer other than pin: rows of pin are
-// still gathered by other workgroups), and (p, q) is summed for alpha.
-template <class T>
-struct EpiCgDir {
-  const T* r;
-  const T* pin;
-  T* pout;
-  T* q;
-  const Ctl* c;
-  FinAlpha f;
-  double b;
-  __device__ void prep() { b = c->beta; }
-  __device__ T xdir(int j) const { return cg_dir<T>(b, pin[j], r[j]); }
-  __device__ void row(int64_t i, T s, T, double* acc) {
-    q[i] = s;
-    const T pi = cg_dir<T>(b, pin[i], r[i]);
-    pout[i] = pi;
-    acc[0] += (double)pi * (double)s;
-  }
-  __device__ void fin(const double* s) const { f(s); }
-};
-
 // x += a p; r -= a q; acc (r,r).  MODE 0: CG (beta, rr, count++ and test),
 // 1: VPCG (count++ and test only).
 template <class T, int MODE>
@@ -144,8 +120,8 @@ struct OpXpby {
   double b;
   __device__ void prep() { b = c->beta; }
   __device__ void elem(int64_t i, double*) {
-    p[i] = cg_dir<T>(b, p[i], r[i]);
-    if (p2) p2[i] = cg_dir<T>(b, p2[i], r2[i]);
+    p[i] = (T)(b * (double)p[i] + (double)r[i]);
+    if (p2) p2[i] = (T)(b * (double)p2[i] + (double)r2[i]);
   }
   __device__ void fin(const double*) {}
 };
@@ -473,21 +449,15 @@ template <class T>
 struct CgLevel : Level {
   MatView<T> A;
   DBuf<T> r, p, q, xown;
-  DBuf<T> p2;  // second direction buffer of the fused path
   T* x = nullptr;
   double* xout = nullptr;
   DBuf<double> r64;  // outer with x0: residual source
   void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf) {
     c = cx; A = a; n = a.n; cfg = cf;
     r.alloc(n); p.alloc(n); q.alloc(n);
-    if (fused()) p2.alloc(n);
     if (sizeof(T) != 8) xown.alloc(n);
     setup_ctl(0, 0);
   }
-  // On workgroup tiles step u > 0 forms its direction inside its SpMV
-  // (EpiCgDir), alternating between p and p2; elsewhere p = beta p + r runs
-  // as its own pass at the end of each step.
-  bool fused() const { return A.kind == 0 && A.wg_tiles; }
   void enqueue_init(const double* b, double* x_out) override {
     xout = x_out;
     x = sizeof(T) == 8 ? (T*)x_out : xown.p;
@@ -500,30 +470,10 @@ struct CgLevel : Level {
     OpCgInit<T, 0> op{src, x, r.p, p.p, nullptr, nullptr, ctl(), !x0given, !x0given};
     launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (x0given ? 24.0 : 32.0) * n);
   }
-  void enqueue_unit(int64_t u) override {
-    if (!fused()) {
-      launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{p.p, q.p, FinAlpha{ctl()}}, gstep(), sizeof(T) * (double)n);
-      launch_map<1>(c, n, OpCgUpdate<T, 0>{x, r.p, p.p, q.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
-      launch_map<0>(c, n, OpXpby<T>{p.p, r.p, nullptr, nullptr, ctl(), 0.0}, gstep(), KS_K_AXPY,
-                    3.0 * sizeof(T) * n);
-      return;
-    }
-    T* pu = (u & 1) ? p2.p : p.p;  // this step's direction (step 0: p from the init)
-    if (u == 0) {
-      launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{p.p, q.p, FinAlpha{ctl()}}, gstep(), sizeof(T) * (double)n);
-    } else {
-      T* pprev = (u & 1) ? p.p : p2.p;
-      const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
-      // SpMV bytes (p_prev gathered as x, q written) + r read and p written
-      const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * n;
-      EpiCgDir<T> e{r.p, pprev, pu, q.p, ctl(), FinAlpha{ctl()}, 0.0};
-      auto kern = k_spmv_stream<T, 1, 1, EpiCgDir<T>>;
-      const int G = grid_rows(c, kern, A.ntiles);
-      klaunch(c, cls, kern, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, pprev, pprev, e, gstep(),
-              red(c), stat(c, cls, bytes));
-      launch_finish<1>(c, G, e, gstep());
-    }
-    launch_map<1>(c, n, OpCgUpdate<T, 0>{x, r.p, pu, q.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+  void enqueue_unit(int64_t) override {
+    launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{p.p, q.p, FinAlpha{ctl()}}, gstep(), sizeof(T) * (double)n);
+    launch_map<1>(c, n, OpCgUpdate<T, 0>{x, r.p, p.p, q.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+    launch_map<0>(c, n, OpXpby<T>{p.p, r.p, nullptr, nullptr, ctl(), 0.0}, gstep(), KS_K_AXPY, 3.0 * sizeof(T) * n);
   }
   int64_t max_units() const override { return cfg.maxloop > 0 ? cfg.maxloop : 0; }
   void enqueue_finish() override {
