@@ -423,7 +423,8 @@ using TileData = QuadTile<T, kBlock, kTileWin>;
 // occupancy each reaches without spilling.
 template <class T, int NX, int NV>
 constexpr int kStreamWaves = sizeof(T) == 4 ? (NV == 0 ? 8 : 7) : (NX == 2 ? 4 : 5);
-// The workgroup kernel's FP32 variants need one more step (72 / 80 VGPRs).
+// The workgroup kernel's FP32 budgets are one step larger (72 / 80 VGPRs):
+// its scaled and projection epilogues take 66-69 VGPRs, and at 64 they spill.
 template <class T, int NX, int NV>
 constexpr int kWgWaves = sizeof(T) == 4 ? (NV == 0 ? 7 : 6) : kStreamWaves<T, NX, NV>;
 
