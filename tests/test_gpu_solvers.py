@@ -80,6 +80,16 @@ def test_gmres_convdiff(ctx, oracle, compat, R):
         assert r["relres_true"] < 2e-10
 
 
+def test_gmres_long_cycle(ctx, oracle):
+    # a cycle longer than the fused projection's 64 columns: steps with more
+    # basis columns fall back to the separate projection kernel (and their
+    # norm step to its own finisher) inside the same cycle
+    ptr, col, val, b, xt = P.convdiff3d(12)
+    x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "gmres", outer_eps=1e-13, outer_maxloop=3000,
+                               outer_restart=100)
+    assert r["status"] == 0 and r["iters"] > 66, r["iters"]
+
+
 def test_gmres_maxloop_exit(ctx, oracle):
     ptr, col, val, b, xt = P.convdiff3d(10)
     x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "gmres", outer_eps=1e-14, outer_maxloop=23,
