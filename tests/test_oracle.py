@@ -9,7 +9,8 @@ cuSPARSE), so each oracle function is pinned by what the mathematics fixes:
   spmv / spmv_f32     exact sums (math.fsum) within the row-sum rounding
                       bound; A*1 closed form; pure sequential order
   transpose           (A^T)^T = A, dense transpose, y^T A x = x^T A^T y,
-                      column counts, rows sorted
+                      column counts, rows sorted; with repeated (row, col)
+                      entries, numpy's stable argsort by column
   dot / norm2         sum_{i<n} i = n(n-1)/2; math.fsum bound
   cgs                 orthogonality, h = V w, ||w_out|| = h[k+1]
   hye                 triangular solve (numpy)
