@@ -90,6 +90,22 @@ def test_gmres_long_cycle(ctx, oracle):
     assert r["status"] == 0 and r["iters"] > 66, r["iters"]
 
 
+def test_solves_identical_run_to_run(ctx):
+    # every reduction, including the norms that the fused GMRES launches sum
+    # in each workgroup, has a fixed order: repeated solves agree bit for bit
+    from solver_cpp_amd import api
+    ptr, col, val, b, xt = P.convdiff3d(14)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    for args, kw in [(("gmres", "NO"), dict(outer_restart=21)),
+                     (("vpgmres", "gmres"), dict(outer_restart=21, inner_maxloop=11, inner_restart=12,
+                                                 inner_precision=32)),
+                     (("bicg", "NO"), {})]:
+        kw = dict(outer_eps=1e-10, outer_maxloop=2000, **kw)
+        x0, r0, h0 = api.solve(ctx, A, b, *args, **kw)
+        x1, r1, h1 = api.solve(ctx, A, b, *args, **kw)
+        assert r0["iters"] == r1["iters"] and np.array_equal(x0, x1) and np.array_equal(h0, h1), args
+
+
 def test_gmres_maxloop_exit(ctx, oracle):
     ptr, col, val, b, xt = P.convdiff3d(10)
     x, r, h, xo, ro, ho = _run(ctx, oracle, ptr, col, val, b, "gmres", outer_eps=1e-14, outer_maxloop=23,
