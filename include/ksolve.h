@@ -66,12 +66,13 @@ typedef enum ks_status {
                                longest row exceeds 256 entries and 16x the mean) */
 #define KS_MAT_VECTOR   64u /* force the lanes-per-row vector SpMV */
 /* CSR-stream tiles: by default the SpMV works on workgroup tiles (256 rows /
- * 2047 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 511) */
+ * 2045 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 509) */
 #define KS_MAT_WG_TILES 128u   /* workgroup tiles (the default) */
 #define KS_MAT_WAVE_TILES 512u /* wavefront tiles */
 #define KS_MAT_BORROW   256u /* with KS_PTR_DEVICE: use the caller's ptr/col/val
                                 in place (no copy); they must outlive the
-                                matrix, and values changed in place are made
+                                matrix, col and val must be 16-byte aligned,
+                                and values changed in place are made
                                 visible with ks_mat_update_values(m, val) */
 
 /* precision selectors */
