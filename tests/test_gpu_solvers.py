@@ -90,6 +90,24 @@ def test_gmres_long_cycle(ctx, oracle):
     assert r["status"] == 0 and r["iters"] > 66, r["iters"]
 
 
+@pytest.mark.parametrize("n", [1, 3, 7])
+def test_gmres_tiny(ctx, oracle, n):
+    # a matrix smaller than one 16-byte chunk of rows: the fused projection's
+    # head/tail rows and a cycle that ends in exact convergence
+    rng = np.random.default_rng(n)
+    ptr = np.zeros(n + 1, np.int32)
+    cols, vals = [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(4.0 if j == i else -1.0 + 0.1 * rng.standard_normal())
+        ptr[i + 1] = len(cols)
+    col, val = np.array(cols, np.int32), np.array(vals)
+    b = rng.standard_normal(n)
+    _run(ctx, oracle, ptr, col, val, b, "gmres", outer_eps=1e-12, outer_maxloop=50, outer_restart=5)
+
+
 def test_solves_identical_run_to_run(ctx):
     # every reduction, including the norms that the fused GMRES launches sum
     # in each workgroup, has a fixed order: repeated solves agree bit for bit
