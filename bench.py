@@ -219,7 +219,7 @@ def suite(ctx, a):
         if nonsym:
             rec["fp64_transpose"] = spmv_gbps(A, n, nnz, 64, trans=True)
         # the other SpMV layouts on the same matrix
-        for lay in ("stream", "wave", "vector", "merge"):
+        for lay in ("stream", "wave", "vector", "merge", "panels"):
             Al = api.Matrix(ctx, P["ptr"], P["col"], P["val"], fp32=True, layout=lay)
             rec[lay] = {"fp64": spmv_gbps(Al, n, nnz, 64), "fp32": spmv_gbps(Al, n, nnz, 32)}
             Al.close()

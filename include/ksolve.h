@@ -69,6 +69,11 @@ typedef enum ks_status {
  * 2045 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 509) */
 #define KS_MAT_WG_TILES 128u   /* workgroup tiles (the default) */
 #define KS_MAT_WAVE_TILES 512u /* wavefront tiles */
+#define KS_MAT_PANELS   1024u /* column panels: the SpMV runs as 8 passes over
+                                 column slices of x, each small enough for an
+                                 XCD's L2 (chosen automatically, with no layout
+                                 flag, for n >= 2^21 rows whose columns spread
+                                 over more than a quarter of n on average) */
 #define KS_MAT_BORROW   256u /* with KS_PTR_DEVICE: use the caller's ptr/col/val
                                 in place (no copy); they must outlive the
                                 matrix, col and val must be 16-byte aligned,

@@ -172,12 +172,14 @@ class Context:
 class Matrix:
     """CSR matrix in HBM (collection's Cval/Ccol/Cptr + CTval/... for A^T)."""
 
-    # SpMV layouts: "auto" (the library's choice: merge-path for heavy-tailed
+    # SpMV layouts: "auto" (the library's choice: column panels for large
+    # matrices whose rows gather x from all over, merge-path for heavy-tailed
     # row lengths, CSR-stream on workgroup tiles otherwise), CSR-stream on
     # workgroup ("stream" / "wg") or wavefront tiles, lanes-per-row vector
-    # kernel, merge-path
+    # kernel, merge-path, column panels
     LAYOUTS = {"auto": 0, "stream": _abi.KS_MAT_WG_TILES, "wg": _abi.KS_MAT_WG_TILES,
-               "wave": _abi.KS_MAT_WAVE_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE}
+               "wave": _abi.KS_MAT_WAVE_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE,
+               "panels": _abi.KS_MAT_PANELS}
 
     def __init__(self, ctx, ptr, col, val, transpose=False, fp32=False, spd=False, layout="auto",
                  borrow=False):

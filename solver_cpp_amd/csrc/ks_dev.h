@@ -193,6 +193,11 @@ struct MatView {
   const int32_t* tiles = nullptr;
   const int32_t* tnz = nullptr;
   T* carry = nullptr;
+  // column panels (kind 3): np sub-matrices over all rows, each holding the
+  // entries whose columns fall in one slice of x; pacc holds row partials
+  int np = 0;
+  const MatView* panel = nullptr;
+  T* pacc = nullptr;
   void set_plan(const Plan& pl) {
     kind = pl.kind;
     L = pl.L;
@@ -203,6 +208,27 @@ struct MatView {
     tnz = ts.tnz.p;
     carry = reinterpret_cast<T*>(pl.carry.p);
   }
+};
+
+// Column panels of one CSR matrix (A or A^T).  On matrices whose rows gather
+// x from all over (random columns), x does not fit an XCD's L2 and every
+// gather is a trip to the MALL.  Split into np panels by column, each SpMV
+// pass gathers from a slice of x that the L2s hold; the passes add their row
+// sums into pacc and the last one applies the epilogue.  Panel p's entries
+// are a CSR over all n rows (ptr from 0), stored from off[p] (a multiple of
+// four entries, so chunk loads stay 16-byte aligned); perm maps each panel
+// entry to its entry of A's value array.
+struct PanelSet {
+  int np = 0;
+  DBuf<int32_t> ptr, col, perm;  // np * (n + 1); panel entries; their A entries
+  DBuf<double> val;
+  DBuf<float> val32;
+  std::vector<int64_t> off;  // np + 1
+  std::vector<Plan> plans;
+  std::vector<MatView<double>> v64;
+  std::vector<MatView<float>> v32;
+  DBuf<double> acc64;
+  DBuf<float> acc32;
 };
 
 }  // namespace ks
@@ -219,6 +245,7 @@ struct ks_mat {
   ks::DBuf<double> tval;
   ks::DBuf<float> tval32;
   ks::Plan plan, tplan;
+  ks::PanelSet panels, tpanels;
   int64_t max_row = 0, max_trow = 0;
 
   template <class T>
@@ -233,6 +260,8 @@ inline ks::MatView<double> ks_mat::view<double>(bool trans) const {
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val.p; }
   v.set_plan(pl);
+  const ks::PanelSet& ps = trans ? tpanels : panels;
+  if (ps.np) { v.np = ps.np; v.panel = ps.v64.data(); v.pacc = ps.acc64.p; v.kind = 3; }
   return v;
 }
 template <>
@@ -243,6 +272,8 @@ inline ks::MatView<float> ks_mat::view<float>(bool trans) const {
   if (trans) { v.ptr = tptr.p; v.col = tcol.p; v.val = tval32.p; }
   else { v.ptr = ptr.p; v.col = col.p; v.val = val32.p; }
   v.set_plan(pl);
+  const ks::PanelSet& ps = trans ? tpanels : panels;
+  if (ps.np && ps.val32.p) { v.np = ps.np; v.panel = ps.v32.data(); v.pacc = ps.acc32.p; v.kind = 3; }
   return v;
 }
 
@@ -378,29 +409,59 @@ int spmv_merge(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const E
   return G;
 }
 
+// One SpMV launch on A's own layout (no finisher); returns the grid size.
+template <class T, int NV, class Epi>
+int spmv_dispatch(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, int cls, double bytes) {
+  if (A.kind == 0) return spmv_stream<T, 1, NV>(c, A, x, x, epi, g, cls, bytes);
+  if (A.kind == 2) return spmv_merge<T, 1, NV>(c, A, x, x, epi, g, cls, bytes);
+  switch (A.L) {
+    case 2: return spmv_L<T, 2, NV>(c, A, x, epi, g, cls, bytes);
+    case 4: return spmv_L<T, 4, NV>(c, A, x, epi, g, cls, bytes);
+    case 8: return spmv_L<T, 8, NV>(c, A, x, epi, g, cls, bytes);
+    case 16: return spmv_L<T, 16, NV>(c, A, x, epi, g, cls, bytes);
+    case 32: return spmv_L<T, 32, NV>(c, A, x, epi, g, cls, bytes);
+    default: return spmv_L<T, 64, NV>(c, A, x, epi, g, cls, bytes);
+  }
+}
+
+// Column-panel passes: every pass but the last stores (first) or adds its
+// row sums into pacc; the last adds pacc to its own and runs the epilogue.
+template <class T>
+struct EpiPanelAcc {
+  T* acc;
+  int first;
+  __device__ void prep() {}
+  __device__ void row(int64_t i, T s, T, double*) { acc[i] = first ? s : acc[i] + s; }
+  __device__ void fin(const double*) const {}
+};
+template <class T, class Epi>
+struct EpiPanelLast {
+  Epi e;
+  const T* acc;
+  __device__ void prep() { e.prep(); }
+  __device__ void row(int64_t i, T s, T s1, double* a) { e.row(i, acc[i] + s, s1, a); }
+  __device__ void fin(const double* s) const { e.fin(s); }
+};
+
 // y-side epilogue of SpMV plus NV reduction values (finished by k_finish).
 template <class T, int NV, class Epi>
 void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, double extra_bytes = 0) {
   const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
   const double bytes = spmv_bytes<T>(A.n, A.nnz) + extra_bytes;
   int G;
-  if (A.kind == 0) {
-    G = spmv_stream<T, 1, NV>(c, A, x, x, epi, g, cls, bytes);
-  } else if (A.kind == 2) {
-    G = spmv_merge<T, 1, NV>(c, A, x, x, epi, g, cls, bytes);
+  if (A.np > 0) {
+    // the bytes of the whole SpMV are counted with the last pass
+    for (int p = 0; p + 1 < A.np; ++p)
+      spmv_dispatch<T, 0>(c, A.panel[p], x, EpiPanelAcc<T>{A.pacc, p == 0}, g, cls, 0.0);
+    G = spmv_dispatch<T, NV>(c, A.panel[A.np - 1], x, EpiPanelLast<T, Epi>{epi, A.pacc}, g, cls, bytes);
   } else {
-    switch (A.L) {
-      case 2: G = spmv_L<T, 2, NV>(c, A, x, epi, g, cls, bytes); break;
-      case 4: G = spmv_L<T, 4, NV>(c, A, x, epi, g, cls, bytes); break;
-      case 8: G = spmv_L<T, 8, NV>(c, A, x, epi, g, cls, bytes); break;
-      case 16: G = spmv_L<T, 16, NV>(c, A, x, epi, g, cls, bytes); break;
-      case 32: G = spmv_L<T, 32, NV>(c, A, x, epi, g, cls, bytes); break;
-      default: G = spmv_L<T, 64, NV>(c, A, x, epi, g, cls, bytes); break;
-    }
+    G = spmv_dispatch<T, NV>(c, A, x, epi, g, cls, bytes);
   }
   if constexpr (NV > 0) launch_finish<NV>(c, G, epi, g);
 }
 
+template <class T>
+struct EpiStore;
 template <class T>
 struct EpiStore2;
 
@@ -408,6 +469,11 @@ struct EpiStore2;
 // with the plan's tiles).
 template <class T>
 void launch_spmv2(ks_ctx* c, const MatView<T>& A, const T* x1, const T* x2, T* y1, T* y2, Gate g) {
+  if (A.np > 0) {  // column panels: two single-vector SpMVs
+    launch_spmv<T, 0>(c, A, x1, EpiStore<T>{y1}, g);
+    launch_spmv<T, 0>(c, A, x2, EpiStore<T>{y2}, g);
+    return;
+  }
   const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * A.n;
   if (A.kind == 2) spmv_merge<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);
   else spmv_stream<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);

@@ -122,6 +122,133 @@ bool merge_choice(uint32_t flags, int64_t n, int64_t nnz, int64_t max_row) {
   return max_row > 256 && (double)max_row > 16.0 * mean;
 }
 
+// Column panels (PanelSet): with KS_MAT_PANELS, and by default (no layout
+// flag) for a matrix with at least 2^21 rows whose rows gather x from all
+// over - the mean column spread of sampled rows above a quarter of n - the
+// SpMV runs as kPanels passes over column slices of x (an x slice of n / 8
+// entries is 4 MB at n = 4M in FP64, an XCD's L2).
+constexpr int kPanels = 8;
+bool panel_choice(uint32_t flags, int64_t n, const int32_t* hp, const int32_t* hc) {
+  if (flags & KS_MAT_PANELS) return true;
+  if (flags & (KS_MAT_VECTOR | KS_MAT_WG_TILES | KS_MAT_WAVE_TILES | KS_MAT_MERGE)) return false;
+  if (n < (int64_t(1) << 21)) return false;
+  double spread = 0.0;
+  int64_t rows = 0;
+  for (int64_t r = 0; r < n; r += 61) {
+    if (hp[r + 1] - hp[r] < 2) continue;
+    int32_t lo = hc[hp[r]], hi = lo;
+    for (int32_t j = hp[r] + 1; j < hp[r + 1]; ++j) {
+      lo = std::min(lo, hc[j]);
+      hi = std::max(hi, hc[j]);
+    }
+    spread += (double)(hi - lo) / (double)n;
+    ++rows;
+  }
+  return rows > 0 && spread / (double)rows > 0.25;
+}
+
+// Panel values from A's values (and their FP32 copy when there is one).
+void refresh_panels(ks_mat* m, PanelSet& ps) {
+  if (!ps.np) return;
+  ks_ctx* c = m->ctx;
+  const int64_t tot = ps.off[ps.np];
+  k_gather<double, double><<<grid_map(tot), kBlock, 0, c->stream>>>(tot, ps.perm.p, m->val.p, ps.val.p);
+  KS_HIPCK(hipGetLastError());
+  if (ps.val32.p) {
+    k_convert<double, float><<<grid_map(tot), kBlock, 0, c->stream>>>(tot, ps.val.p, ps.val32.p, Gate{});
+    KS_HIPCK(hipGetLastError());
+  }
+}
+
+// FP32 panel values and views (after the matrix has its FP32 copy).
+void panels_fp32(ks_mat* m, PanelSet& ps) {
+  if (!ps.np || ps.val32.p) return;
+  ks_ctx* c = m->ctx;
+  const int64_t tot = ps.off[ps.np];
+  ps.val32.alloc(tot);
+  ps.acc32.alloc(m->n);
+  k_convert<double, float><<<grid_map(tot), kBlock, 0, c->stream>>>(tot, ps.val.p, ps.val32.p, Gate{});
+  KS_HIPCK(hipGetLastError());
+  ps.v32.assign(ps.np, MatView<float>{});
+  for (int p = 0; p < ps.np; ++p) {
+    MatView<float>& v = ps.v32[p];
+    v.n = m->n; v.nnz = ps.v64[p].nnz;
+    v.ptr = ps.v64[p].ptr; v.col = ps.v64[p].col; v.val = ps.val32.p + ps.off[p];
+    v.set_plan(ps.plans[p]);
+  }
+}
+
+// Split the CSR (hp, hc: host row pointers and columns of A or A^T) into
+// column panels.  src_perm (device) maps the CSR's entries to A's entries
+// (A^T: tperm), or is null for A itself.
+void build_panels(ks_mat* m, PanelSet& ps, const int32_t* hp, const int32_t* hc, const int32_t* src_perm,
+                  uint32_t flags) {
+  ks_ctx* c = m->ctx;
+  const int64_t n = m->n;
+  const int P = kPanels;
+  auto panel_of = [&](int32_t col) { return (int)((int64_t)col * P / n); };
+  std::vector<int32_t> hpp((size_t)P * (n + 1), 0);
+  for (int64_t r = 0; r < n; ++r)
+    for (int32_t j = hp[r]; j < hp[r + 1]; ++j) hpp[(size_t)panel_of(hc[j]) * (n + 1) + r + 1]++;
+  std::vector<int64_t> off(P + 1, 0);
+  std::vector<int64_t> maxrow(P, 0);
+  for (int p = 0; p < P; ++p) {
+    int32_t* q = hpp.data() + (size_t)p * (n + 1);
+    for (int64_t r = 0; r < n; ++r) {
+      maxrow[p] = std::max<int64_t>(maxrow[p], q[r + 1]);
+      q[r + 1] += q[r];
+    }
+    off[p + 1] = (off[p] + q[n] + 3) / 4 * 4;  // 16-byte aligned panel starts
+  }
+  const int64_t tot = std::max<int64_t>(off[P], 1);
+  std::vector<int32_t> perm(tot, 0), pcol(tot, 0);
+  std::vector<int64_t> pos(P);
+  for (int64_t r = 0; r < n; ++r) {
+    for (int p = 0; p < P; ++p) pos[p] = off[p] + hpp[(size_t)p * (n + 1) + r];
+    for (int32_t j = hp[r]; j < hp[r + 1]; ++j) {
+      const int p = panel_of(hc[j]);
+      perm[pos[p]] = j;
+      pcol[pos[p]] = hc[j];
+      ++pos[p];
+    }
+  }
+  hipStream_t s = c->stream;
+  ps.ptr.alloc(hpp.size());
+  ps.col.alloc(tot);
+  ps.perm.alloc(tot);
+  ps.val.alloc(tot);
+  ps.acc64.alloc(n);
+  KS_HIPCK(hipMemcpyAsync(ps.ptr.p, hpp.data(), sizeof(int32_t) * hpp.size(), hipMemcpyHostToDevice, s));
+  KS_HIPCK(hipMemcpyAsync(ps.col.p, pcol.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, s));
+  if (src_perm) {
+    DBuf<int32_t> tmp(tot);
+    KS_HIPCK(hipMemcpyAsync(tmp.p, perm.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, s));
+    k_gather<int32_t, int32_t><<<grid_map(tot), kBlock, 0, s>>>(tot, tmp.p, src_perm, ps.perm.p);
+    KS_HIPCK(hipGetLastError());
+    KS_HIPCK(hipStreamSynchronize(s));
+  } else {
+    KS_HIPCK(hipMemcpyAsync(ps.perm.p, perm.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, s));
+  }
+  ps.off = off;
+  ps.plans.clear();
+  ps.plans.resize(P);
+  ps.v64.assign(P, MatView<double>{});
+  const uint32_t lay = flags & ~(uint32_t)KS_MAT_PANELS;
+  for (int p = 0; p < P; ++p) {
+    const int32_t* q = hpp.data() + (size_t)p * (n + 1);
+    const int64_t np_nnz = q[n];
+    make_plan(c, ps.plans[p], q, n, np_nnz, (lay & KS_MAT_VECTOR) != 0, tile_choice(lay),
+              merge_choice(lay, n, np_nnz, maxrow[p]));
+    MatView<double>& v = ps.v64[p];
+    v.n = n; v.nnz = np_nnz;
+    v.ptr = ps.ptr.p + (size_t)p * (n + 1); v.col = ps.col.p + off[p]; v.val = ps.val.p + off[p];
+    v.set_plan(ps.plans[p]);
+  }
+  ps.np = P;
+  refresh_panels(m, ps);
+  KS_HIPCK(hipStreamSynchronize(s));  // the host arrays above are freed on return
+}
+
 // A^T row segments longer than kSortMax: bitonic sorts of kSortMax-entry
 // chunks, then pairwise merges of sorted runs (width kSortMax, 2 kSortMax,
 // ...) that alternate between A^T's arrays and a flat staging buffer.
@@ -223,6 +350,12 @@ void build_transpose(ks_mat* m) {
   make_plan(c, m->tplan, htp.data(), n, nnz, (m->flags & KS_MAT_VECTOR) != 0, tile_choice(m->flags),
             merge_choice(m->flags, n, nnz, mx));
   m->has_t = true;
+  if (m->panels.np && nnz > 0) {  // A^T gathers as randomly as A
+    std::vector<int32_t> htc(nnz);
+    KS_HIPCK(hipMemcpyAsync(htc.data(), m->tcol.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
+    KS_HIPCK(hipStreamSynchronize(s));
+    build_panels(m, m->tpanels, htp.data(), htc.data(), m->tperm.p, m->flags);
+  }
 }
 
 void ensure_fp32(ks_mat* m) {
@@ -238,6 +371,8 @@ void ensure_fp32(ks_mat* m) {
     k_convert<double, float><<<grid_map(nnz), kBlock, 0, c->stream>>>(nnz, m->tval.p, m->tval32.p, Gate{});
     KS_HIPCK(hipGetLastError());
   }
+  panels_fp32(m, m->panels);
+  panels_fp32(m, m->tpanels);
 }
 
 // ------------------------------------------------------------- levels ---
@@ -586,6 +721,8 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   m->max_row = mx;
   make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0, tile_choice(flags),
               merge_choice(flags, n, nnz, mx));
+  if (nnz > 0 && panel_choice(flags, n, hp.data(), hc.data()))
+    build_panels(m.get(), m->panels, hp.data(), hc.data(), nullptr, flags);
   if (flags & KS_MAT_TRANSPOSE) build_transpose(m.get());
   if (flags & KS_MAT_FP32_COPY) ensure_fp32(m.get());
   KS_HIPCK(hipStreamSynchronize(c->stream));
@@ -611,6 +748,8 @@ ks_status ks_mat_update_values(ks_mat* m, const double* val, uint32_t flags) {
     k_gather<double, double><<<grid_map(m->nnz), kBlock, 0, c->stream>>>(m->nnz, m->tperm.p, m->val.p, m->tval.p);
     KS_HIPCK(hipGetLastError());
   }
+  refresh_panels(m, m->panels);
+  refresh_panels(m, m->tpanels);
   if (m->val32.p) {
     k_convert<double, float><<<grid_map(m->nnz), kBlock, 0, c->stream>>>(m->nnz, m->val.p, m->val32.p, Gate{});
     KS_HIPCK(hipGetLastError());
@@ -652,7 +791,7 @@ ks_status ks_mat_info(const ks_mat* m, int64_t* n, int64_t* nnz, int32_t* kind, 
   if (!m) return KS_ERR_BAD_INPUT;
   if (n) *n = m->n;
   if (nnz) *nnz = m->nnz;
-  if (kind) *kind = m->plan.kind;
+  if (kind) *kind = m->panels.np ? 3 : m->plan.kind;
   if (lanes) *lanes = m->plan.L;
   return KS_OK;
 }

@@ -21,7 +21,7 @@ def _mat(ctx, ptr, col, val, **kw):
     return api.Matrix(ctx, ptr, col, val, **kw)
 
 
-LAYOUTS = ["auto", "stream", "wg", "wave", "vector", "merge"]
+LAYOUTS = ["auto", "stream", "wg", "wave", "vector", "merge", "panels"]
 
 
 def _auto_kind(ptr):
@@ -42,7 +42,7 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     n = len(ptr) - 1
     x = np.random.default_rng(7).standard_normal(n)
     A = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
-    kinds = {"auto": _auto_kind(ptr), "stream": 0, "wg": 0, "wave": 0, "vector": 1, "merge": 2}
+    kinds = {"auto": _auto_kind(ptr), "stream": 0, "wg": 0, "wave": 0, "vector": 1, "merge": 2, "panels": 3}
     assert A.info()["spmv_kind"] == kinds[layout]
     m = np.diff(ptr).max()
     y = A.spmv(x)
@@ -158,7 +158,7 @@ def _dense_column_csr(n, seed):
     return ptr, col, val
 
 
-@pytest.mark.parametrize("layout", ["auto", "stream"])
+@pytest.mark.parametrize("layout", ["auto", "stream", "panels"])
 def test_transpose_dense_columns(ctx, oracle, layout):
     ptr, col, val = _dense_column_csr(100_000, 17)
     n = len(ptr) - 1
@@ -252,7 +252,7 @@ def test_cgs_step(ctx, oracle, k, n):
     assert np.abs(V @ wg).max() <= 1e-12 * np.linalg.norm(w)
 
 
-@pytest.mark.parametrize("layout", ["stream", "wg", "wave", "merge"])
+@pytest.mark.parametrize("layout", ["stream", "wg", "wave", "merge", "panels"])
 @pytest.mark.parametrize("k", [1, 2, 4])
 def test_kskip_basis_and_gram(ctx, oracle, k, layout):
     ptr, col, val, _, _ = P.convdiff3d(10, pe=0.0)

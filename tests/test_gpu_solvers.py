@@ -262,7 +262,7 @@ def test_adjoint_small(ctx, oracle):
     assert np.allclose(gv, -lam[rows] * x[col], rtol=0, atol=1e-9 * np.abs(lam).max() * np.abs(x).max())
 
 
-@pytest.mark.parametrize("layout", ["wg", "wave", "vector", "merge"])
+@pytest.mark.parametrize("layout", ["wg", "wave", "vector", "merge", "panels"])
 def test_solvers_on_every_spmv_layout(ctx, oracle, layout):
     # the fused SpMV epilogues (residual + norm, dot) and the two-input
     # k-skip basis on each SpMV layout give the default layout's results.
