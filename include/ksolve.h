@@ -72,8 +72,9 @@ typedef enum ks_status {
 #define KS_MAT_PANELS   1024u /* column panels: the SpMV runs as 8 passes over
                                  column slices of x, each small enough for an
                                  XCD's L2 (chosen automatically, with no layout
-                                 flag, for n >= 2^21 rows whose columns spread
-                                 over more than a quarter of n on average) */
+                                 flag, for n >= 2^21 rows when over half the
+                                 entries lie more than n/8 columns from their
+                                 row) */
 #define KS_MAT_BORROW   256u /* with KS_PTR_DEVICE: use the caller's ptr/col/val
                                 in place (no copy); they must outlive the
                                 matrix, col and val must be 16-byte aligned,

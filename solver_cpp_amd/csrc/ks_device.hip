@@ -123,28 +123,25 @@ bool merge_choice(uint32_t flags, int64_t n, int64_t nnz, int64_t max_row) {
 }
 
 // Column panels (PanelSet): with KS_MAT_PANELS, and by default (no layout
-// flag) for a matrix with at least 2^21 rows whose rows gather x from all
-// over - the mean column spread of sampled rows above a quarter of n - the
-// SpMV runs as kPanels passes over column slices of x (an x slice of n / 8
-// entries is 4 MB at n = 4M in FP64, an XCD's L2).
+// flag) for a matrix with at least 2^21 rows whose gathers are mostly far
+// from the diagonal - over half of the entries of sampled rows more than n/8
+// columns from their row, as with random columns or a scrambled ordering (a
+// banded matrix, or one whose rows add a few far columns to a local band,
+// gathers mostly near its rows and keeps one pass) - the SpMV runs as kPanels
+// passes over column slices of x (an x slice of n / 8 entries is 4 MB at
+// n = 4M in FP64, an XCD's L2).
 constexpr int kPanels = 8;
 bool panel_choice(uint32_t flags, int64_t n, const int32_t* hp, const int32_t* hc) {
   if (flags & KS_MAT_PANELS) return true;
   if (flags & (KS_MAT_VECTOR | KS_MAT_WG_TILES | KS_MAT_WAVE_TILES | KS_MAT_MERGE)) return false;
   if (n < (int64_t(1) << 21)) return false;
-  double spread = 0.0;
-  int64_t rows = 0;
-  for (int64_t r = 0; r < n; r += 61) {
-    if (hp[r + 1] - hp[r] < 2) continue;
-    int32_t lo = hc[hp[r]], hi = lo;
-    for (int32_t j = hp[r] + 1; j < hp[r + 1]; ++j) {
-      lo = std::min(lo, hc[j]);
-      hi = std::max(hi, hc[j]);
+  int64_t far = 0, all = 0;
+  for (int64_t r = 0; r < n; r += 61)
+    for (int32_t j = hp[r]; j < hp[r + 1]; ++j) {
+      far += std::llabs((int64_t)hc[j] - r) > n / 8;
+      ++all;
     }
-    spread += (double)(hi - lo) / (double)n;
-    ++rows;
-  }
-  return rows > 0 && spread / (double)rows > 0.25;
+  return all > 0 && 2 * far > all;
 }
 
 // Panel values from A's values (and their FP32 copy when there is one).
