@@ -4,6 +4,7 @@
 #include <chrono>
 #include <deque>
 #include <memory>
+#include <thread>
 
 #include "ks_levels.h"
 
@@ -175,6 +176,20 @@ void panels_fp32(ks_mat* m, PanelSet& ps) {
   }
 }
 
+// fn(lo, hi) over [0, n) in contiguous row blocks, on up to 16 host threads.
+template <class F>
+void host_rows(int64_t n, F fn) {
+  const int64_t T = std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(T, n / 65536));
+  if (nt == 1) {
+    fn(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+  for (auto& x : th) x.join();
+}
+
 // Split the CSR (hp, hc: host row pointers and columns of A or A^T) into
 // column panels.  src_perm (device) maps the CSR's entries to A's entries
 // (A^T: tperm), or is null for A itself.
@@ -184,31 +199,43 @@ void build_panels(ks_mat* m, PanelSet& ps, const int32_t* hp, const int32_t* hc,
   const int64_t n = m->n;
   const int P = kPanels;
   auto panel_of = [&](int32_t col) { return (int)((int64_t)col * P / n); };
+  // per-row counts (each row written by one thread), then per-panel scans
   std::vector<int32_t> hpp((size_t)P * (n + 1), 0);
-  for (int64_t r = 0; r < n; ++r)
-    for (int32_t j = hp[r]; j < hp[r + 1]; ++j) hpp[(size_t)panel_of(hc[j]) * (n + 1) + r + 1]++;
-  std::vector<int64_t> off(P + 1, 0);
+  host_rows(n, [&](int64_t lo, int64_t hi) {
+    for (int64_t r = lo; r < hi; ++r)
+      for (int32_t j = hp[r]; j < hp[r + 1]; ++j) hpp[(size_t)panel_of(hc[j]) * (n + 1) + r + 1]++;
+  });
   std::vector<int64_t> maxrow(P, 0);
-  for (int p = 0; p < P; ++p) {
-    int32_t* q = hpp.data() + (size_t)p * (n + 1);
-    for (int64_t r = 0; r < n; ++r) {
-      maxrow[p] = std::max<int64_t>(maxrow[p], q[r + 1]);
-      q[r + 1] += q[r];
-    }
-    off[p + 1] = (off[p] + q[n] + 3) / 4 * 4;  // 16-byte aligned panel starts
+  {
+    std::vector<std::thread> th;
+    for (int p = 0; p < P; ++p)
+      th.emplace_back([&, p] {
+        int32_t* q = hpp.data() + (size_t)p * (n + 1);
+        for (int64_t r = 0; r < n; ++r) {
+          maxrow[p] = std::max<int64_t>(maxrow[p], q[r + 1]);
+          q[r + 1] += q[r];
+        }
+      });
+    for (auto& x : th) x.join();
   }
+  std::vector<int64_t> off(P + 1, 0);
+  for (int p = 0; p < P; ++p)
+    off[p + 1] = (off[p] + hpp[(size_t)p * (n + 1) + n] + 3) / 4 * 4;  // 16-byte aligned panel starts
   const int64_t tot = std::max<int64_t>(off[P], 1);
   std::vector<int32_t> perm(tot, 0), pcol(tot, 0);
-  std::vector<int64_t> pos(P);
-  for (int64_t r = 0; r < n; ++r) {
-    for (int p = 0; p < P; ++p) pos[p] = off[p] + hpp[(size_t)p * (n + 1) + r];
-    for (int32_t j = hp[r]; j < hp[r + 1]; ++j) {
-      const int p = panel_of(hc[j]);
-      perm[pos[p]] = j;
-      pcol[pos[p]] = hc[j];
-      ++pos[p];
+  // each row's entries go to their panels in CSR order (rows are disjoint)
+  host_rows(n, [&](int64_t lo, int64_t hi) {
+    std::vector<int64_t> pos(P);
+    for (int64_t r = lo; r < hi; ++r) {
+      for (int p = 0; p < P; ++p) pos[p] = off[p] + hpp[(size_t)p * (n + 1) + r];
+      for (int32_t j = hp[r]; j < hp[r + 1]; ++j) {
+        const int p = panel_of(hc[j]);
+        perm[pos[p]] = j;
+        pcol[pos[p]] = hc[j];
+        ++pos[p];
+      }
     }
-  }
+  });
   hipStream_t s = c->stream;
   ps.ptr.alloc(hpp.size());
   ps.col.alloc(tot);
