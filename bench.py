@@ -59,6 +59,9 @@ def parse():
                     help="also run the per-configuration SpMV, parity-solve and adjoint measurements "
                          "(one JSON line each, after the headline line)")
     ap.add_argument("--fd-dirs", type=int, default=64, help="finite-difference directions per adjoint case")
+    ap.add_argument("--spmv-config", default=None,
+                    help="only the SpMV measurement of one configuration (its default layout, A and A^T), "
+                         "one JSON line, no headline: e.g. zipf, for a profiler run")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB of one plain device copy at the end (counter calibration)")
     return ap.parse_args()
@@ -73,6 +76,19 @@ def main():
     import torch
     from solver_cpp_amd import api, gen
     from solver_cpp_amd._abi import KERNEL_CLASSES
+
+    if a.spmv_config:
+        P = gen.problem(a.spmv_config)
+        n, nnz = len(P["ptr"]) - 1, len(P["col"])
+        ctx = api.Context(0)
+        A = api.Matrix(ctx, P["ptr"], P["col"], P["val"], fp32=True, transpose=True)
+        print(json.dumps({"suite": "spmv", "config": a.spmv_config, "n": n, "nnz": nnz,
+                          "spmv_kind": A.info()["spmv_kind"], "fp64": spmv_gbps(A, n, nnz, 64),
+                          "fp32": spmv_gbps(A, n, nnz, 32), "fp64_transpose": spmv_gbps(A, n, nnz, 64, trans=True)}),
+              flush=True)
+        A.close()
+        ctx.close()
+        return
 
     g = a.grid
     P = gen.problem("poisson3d", scale=g)
