@@ -349,19 +349,27 @@ void build_transpose(ks_mat* m) {
     KS_HIPCK(hipGetLastError());
   }
   // per-segment sort makes the result independent of the scatter order
-  DBuf<int32_t> longrows(n + 1);
+  DBuf<int32_t> longrows(n + 1), midrows(n + 1);
   KS_HIPCK(hipMemsetAsync(longrows.p + n, 0, sizeof(int32_t), s));
-  k_sort_small<<<grid_map(n), kBlock, 0, s>>>(n, m->tptr.p, m->tcol.p, m->tperm.p, longrows.p, longrows.p + n);
+  KS_HIPCK(hipMemsetAsync(midrows.p + n, 0, sizeof(int32_t), s));
+  k_sort_small<<<grid_map(n), kBlock, 0, s>>>(n, m->tptr.p, m->tcol.p, m->tperm.p, longrows.p, longrows.p + n,
+                                              midrows.p, midrows.p + n);
   KS_HIPCK(hipGetLastError());
   std::vector<int32_t> htp(n + 1);
-  int32_t nlong = 0;
+  int32_t nlong = 0, nmid = 0;
   KS_HIPCK(hipMemcpyAsync(htp.data(), m->tptr.p, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, s));
   KS_HIPCK(hipMemcpyAsync(&nlong, longrows.p + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  KS_HIPCK(hipMemcpyAsync(&nmid, midrows.p + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   KS_HIPCK(hipStreamSynchronize(s));
   KS_CHECK(htp[n] == nnz, KS_ERR_BAD_INPUT, "transpose: column counts do not add up to nnz");
   int64_t mx = 0;
   for (int64_t i = 0; i < n; ++i) mx = std::max<int64_t>(mx, htp[i + 1] - htp[i]);
   m->max_trow = mx;
+  if (nmid > 0) {
+    k_sort_mid<<<(nmid + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(nmid, m->tptr.p, midrows.p, m->tcol.p,
+                                                                         m->tperm.p);
+    KS_HIPCK(hipGetLastError());
+  }
   if (nlong > 0) {
     k_sort_long<<<nlong, 1024, 0, s>>>(m->tptr.p, longrows.p, m->tcol.p, m->tperm.p);
     KS_HIPCK(hipGetLastError());

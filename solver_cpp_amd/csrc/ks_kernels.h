@@ -1323,17 +1323,30 @@ __global__ __launch_bounds__(kBlock) void k_tscatter(int64_t n, const int32_t* _
 // (tperm): entries of one column of A taken in CSR order come in row order,
 // ties between repeated (row, column) entries by their position in the row,
 // so the result is the stable column-major order of a counting-sort
-// transpose, duplicates included.  One thread per segment of <= 64 entries
-// (insertion sort); longer segments are listed for k_sort_long.
+// transpose, duplicates included.  One thread per segment of <= 16 entries
+// (insertion sort); segments of 17..64 entries are listed for k_sort_mid,
+// longer ones for k_sort_long.
+// Appends v to list for the active lanes with pred set: one atomic per
+// wavefront (a counter hit once per lane serialises at the L2).  Called by
+// all active lanes together.
+__device__ __forceinline__ void wave_append(bool pred, int32_t v, int32_t* list, int32_t* cnt) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63, leader = __ffsll((unsigned long long)m) - 1;
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1))] = v;
+}
+
 __global__ void k_sort_small(int64_t n, const int32_t* __restrict__ tptr, int32_t* tcol, int32_t* tperm,
-                             int32_t* long_rows, int32_t* n_long) {
+                             int32_t* long_rows, int32_t* n_long, int32_t* mid_rows, int32_t* n_mid) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
     const int s = tptr[r], e = tptr[r + 1];
-    if (e - s > 64) {
-      long_rows[atomicAdd(n_long, 1)] = (int32_t)r;
-      continue;
-    }
+    wave_append(e - s > 64, (int32_t)r, long_rows, n_long);
+    wave_append(e - s > 16 && e - s <= 64, (int32_t)r, mid_rows, n_mid);
+    if (e - s > 16) continue;
     for (int a = s + 1; a < e; ++a) {
       const int32_t kc = tcol[a];
       const int32_t kp = tperm[a];
@@ -1346,6 +1359,38 @@ __global__ void k_sort_small(int64_t n, const int32_t* __restrict__ tptr, int32_
       tcol[b + 1] = kc;
       tperm[b + 1] = kp;
     }
+  }
+}
+
+// One wavefront per listed segment of 17..64 entries: a 64-lane bitonic sort
+// by tperm in registers (shuffles), entry l on lane l, the unused lanes
+// holding the largest key.  Segments are taken four to a workgroup.
+__global__ __launch_bounds__(kBlock) void k_sort_mid(int32_t nmid, const int32_t* __restrict__ tptr,
+                                                     const int32_t* __restrict__ mid_rows, int32_t* tcol,
+                                                     int32_t* tperm) {
+  const int lane = threadIdx.x & 63;
+  const int32_t q = (int32_t)blockIdx.x * (kBlock / 64) + (int32_t)(threadIdx.x >> 6);
+  if (q >= nmid) return;  // the whole wavefront
+  const int32_t r = mid_rows[q];
+  const int s = tptr[r], len = tptr[r + 1] - s;
+  int32_t key = lane < len ? tperm[s + lane] : 0x7fffffff;
+  int32_t pay = lane < len ? tcol[s + lane] : 0;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int32_t pk = __shfl_xor(key, j, 64);
+      const int32_t pp = __shfl_xor(pay, j, 64);
+      const bool lower = (lane & j) == 0, up = (lane & k) == 0;
+      if (lower == up ? pk < key : pk > key) {
+        key = pk;
+        pay = pp;
+      }
+    }
+  }
+  if (lane < len) {
+    tperm[s + lane] = key;
+    tcol[s + lane] = pay;
   }
 }
 

@@ -177,6 +177,30 @@ def test_transpose_dense_columns(ctx, oracle, layout):
     A.close()
 
 
+def test_transpose_segment_sizes(ctx, oracle):
+    # A^T rows of every length 0..140 (each sort path and the limits between
+    # them: 16/17 and 64/65 entries), rows drawn with repeats
+    rng = np.random.default_rng(23)
+    n = 4000
+    lens = np.arange(n) % 141
+    rows = [[] for _ in range(n)]
+    for j, L in enumerate(lens):
+        for i in rng.integers(0, n, L):
+            rows[i].append(j)
+    for r in rows:
+        rng.shuffle(r)
+    ptr = np.zeros(n + 1, np.int32)
+    ptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.fromiter((c for r in rows for c in r), np.int32, count=int(ptr[-1]))
+    val = rng.standard_normal(len(col))
+    A = _mat(ctx, ptr, col, val, transpose=True, layout="stream")
+    tp, tc, tv = A.transpose_csr()
+    otp, otc, otv = oracle.transpose(ptr, col, val)
+    assert np.array_equal(np.diff(otp), lens)
+    assert np.array_equal(tp, otp) and np.array_equal(tc, otc) and np.array_equal(tv, otv)
+    A.close()
+
+
 def test_borrowed_matrix(ctx, oracle):
     # KS_MAT_BORROW: the caller's device CSR is used in place
     import torch
