@@ -570,6 +570,16 @@ struct EpiStore {
   __device__ void row(int64_t i, T s, T, double*) { y[i] = s; }
   __device__ void fin(const double*) const {}
 };
+// y = A x streamed out with non-temporal stores: the standalone SpMV, whose
+// result goes back to the caller.  Inside the solvers the next kernel reads y
+// at once and plain stores (EpiStore) keep it in the memory-side cache.
+template <class T>
+struct EpiStream {
+  T* y;
+  __device__ void prep() {}
+  __device__ void row(int64_t i, T s, T, double*) { __builtin_nontemporal_store(s, y + i); }
+  __device__ void fin(const double*) const {}
+};
 // y = (A x) * r (r: device FP64 reciprocal of the basis norm, formed by the
 // norm finisher; rounded to T once per thread and held in scalar registers,
 // as it is uniform)

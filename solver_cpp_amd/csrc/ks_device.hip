@@ -838,13 +838,13 @@ ks_status ks_spmv(ks_mat* m, int32_t trans, int32_t precision, const void* x, vo
     ensure_fp32(m);
     In<float> xi(c, (const float*)x, m->n, flags);
     Out<float> yo(c, (float*)y, m->n, flags);
-    launch_spmv<float, 0>(c, m->view<float>(trans != 0), xi.p, EpiStore<float>{yo.p}, Gate{});
+    launch_spmv<float, 0>(c, m->view<float>(trans != 0), xi.p, EpiStream<float>{yo.p}, Gate{});
     yo.copy_back(c);
   } else {
     KS_CHECK(precision == KS_F64, KS_ERR_BAD_INPUT, "precision must be 64 or 32");
     In<double> xi(c, (const double*)x, m->n, flags);
     Out<double> yo(c, (double*)y, m->n, flags);
-    launch_spmv<double, 0>(c, m->view<double>(trans != 0), xi.p, EpiStore<double>{yo.p}, Gate{});
+    launch_spmv<double, 0>(c, m->view<double>(trans != 0), xi.p, EpiStream<double>{yo.p}, Gate{});
     yo.copy_back(c);
   }
   KS_HIPCK(hipStreamSynchronize(c->stream));
@@ -866,9 +866,9 @@ ks_status ks_spmv_bench(ks_mat* m, int32_t trans, int32_t precision, const void*
   KS_HIPCK(hipEventRecord(a, c->stream));
   for (int r = 0; r < reps; ++r) {
     if (precision == KS_F32)
-      launch_spmv<float, 0>(c, m->view<float>(trans != 0), (const float*)x, EpiStore<float>{(float*)y}, Gate{});
+      launch_spmv<float, 0>(c, m->view<float>(trans != 0), (const float*)x, EpiStream<float>{(float*)y}, Gate{});
     else
-      launch_spmv<double, 0>(c, m->view<double>(trans != 0), (const double*)x, EpiStore<double>{(double*)y}, Gate{});
+      launch_spmv<double, 0>(c, m->view<double>(trans != 0), (const double*)x, EpiStream<double>{(double*)y}, Gate{});
   }
   KS_HIPCK(hipEventRecord(b, c->stream));
   KS_HIPCK(hipEventSynchronize(b));
