@@ -170,7 +170,10 @@ ks_status ks_mat_get_transpose(ks_mat* m, int32_t* tptr, int32_t* tcol,
                                double* tval, uint32_t flags);
 /* Destroy every matrix of a context before the context itself. */
 void ks_mat_destroy(ks_mat* m);
-/* n, nnz and the SpMV variant chosen (0 vector, 1 merge-path) with its lanes/row */
+/* n, nnz and the SpMV variant chosen: spmv_kind 0 CSR-stream tiles (workgroup
+ * or wavefront), 1 lanes-per-row vector kernel, 2 merge path, 3 column panels;
+ * lanes_per_row is the vector kernel's lanes per row (the smallest power of
+ * two >= the mean row length, in [2, 64]), reported for every kind. */
 ks_status ks_mat_info(const ks_mat* m, int64_t* n, int64_t* nnz,
                       int32_t* spmv_kind, int32_t* lanes_per_row);
 
