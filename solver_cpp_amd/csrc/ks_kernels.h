@@ -772,18 +772,43 @@ __global__ __launch_bounds__(kBlock) void k_merge_carry(int64_t n, int64_t nseg,
   RowBlocks rbs{nseg};
   int64_t s, end, step;
   rbs.range(s, end, step, w, kBlock / 64);
+  // a wavefront's segments follow one another: the next segment's bounds and
+  // row start are loaded while the current one is summed, so each segment
+  // waits for its entries and gathers only
+  int32_t r = 0, js = 0, je = 0, pr = 0;
+  if (s < end) {
+    r = segr[s + 1]; js = segj[s]; je = segj[s + 1];
+    if (r < n) pr = ptr[r];
+  }
   for (; s < end; s += step) {
-    const int32_t r = segr[s + 1], js = segj[s], je = segj[s + 1];
+    const int64_t sn = s + step;
+    int32_t nr = 0, njs = 0, nje = 0, npr = 0;
+    if (sn < end) { nr = segr[sn + 1]; njs = segj[sn]; nje = segj[sn + 1]; }
     T s0 = 0, s1 = 0;
     if (r < n) {
-      const int32_t a = ptr[r] > js ? ptr[r] : js;
-      for (int32_t k = a + lane; k < je; k += 64) {
+      const int32_t a = pr > js ? pr : js;
+      int32_t k = a + lane;
+      // two entries per lane in flight; each lane still adds its entries in
+      // increasing k, as one at a time would
+      for (; k + 64 < je; k += 128) {
+        const T v0 = ld_stream(val + k), v1 = ld_stream(val + k + 64);
+        const int c0 = ld_stream(col + k), c1 = ld_stream(col + k + 64);
+        const T g0 = x0[c0], g1 = x0[c1];
+        T h0 = 0, h1 = 0;
+        if (NX == 2) { h0 = x1[c0]; h1 = x1[c1]; }
+        s0 += v0 * g0;
+        if (NX == 2) s1 += v0 * h0;
+        s0 += v1 * g1;
+        if (NX == 2) s1 += v1 * h1;
+      }
+      if (k < je) {
         const T v = ld_stream(val + k);
         const int c = ld_stream(col + k);
         s0 += v * x0[c];
         if (NX == 2) s1 += v * x1[c];
       }
     }
+    if (sn < end && nr < n) npr = ptr[nr];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       s0 += __shfl_xor(s0, off, 64);
@@ -793,6 +818,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_carry(int64_t n, int64_t nseg,
       carry[s] = s0;
       if (NX == 2) carry[nseg + s] = s1;
     }
+    r = nr; js = njs; je = nje; pr = npr;
   }
 }
 
