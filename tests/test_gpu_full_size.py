@@ -12,7 +12,8 @@ that hold at any size (the oracle's own runs are kept to small sizes).
   (cudaFunction.cu:896-931) at the full sizes of the 160^3 Poisson, 192^3
   convection-diffusion and 4,000,000-row Zipf matrices (the last on column
   panels): FP64 and FP32 against scipy's FP64 product within the row-sum
-  rounding bound, and the adjoint identity (z, A x) = (A^T z, x).
+  rounding bound, the adjoint identity (z, A x) = (A^T z, x), and A^T equal
+  entry for entry to scipy's stable CSC -> CSR conversion.
 """
 import math
 
@@ -86,7 +87,11 @@ def test_spmv_full_size(ctx, cfg):
     # A^T against scipy's transposed product, and the adjoint identity
     mt = int(np.bincount(col, minlength=n).max())
     yt = A.spmv(z, trans=True)
+    # scipy's CSC -> CSR conversion is a stable counting sort: the A^T the
+    # device built must equal it entry for entry (repeated entries kept)
     St = S.T.tocsr()
+    tp, tc, tv = A.transpose_csr()
+    assert np.array_equal(tp, St.indptr) and np.array_equal(tc, St.indices) and np.array_equal(tv, St.data)
     assert np.all(np.abs(yt - St @ z) <= 4 * mt * EPS64 * (abs(St) @ np.abs(z)) + 1e-300)
     # exact sums of the rounded products: what is left is the two SpMVs'
     # rounding, (m + 1) and (mt + 1) eps times sum |z_i a_ij x_j|, and the
