@@ -74,6 +74,11 @@ struct Out {
 };
 
 double g_kskip_oob = 0.0;  // value read past eta[] (kskipcg.cpp:284, j=2k-1)
+// Trace of k-skip CG (for the tests' step-by-step checks): when enabled, each
+// block appends its starting r and p (n values each), then (alpha, beta) of
+// each of its k+1 inner steps.
+bool g_kskip_trace_on = false;
+std::vector<double> g_kskip_trace;
 
 // ---------------------------------------------------------------- BLAS ----
 // blas::dot (blas.cpp:149-162): emulated static partition over nchunk
@@ -531,9 +536,17 @@ void kskipcg(const Ws<T>& W, const T* b, T* x, const Cfg& c, Out& o) {
     double gamma = dot(n, r.data(), r.data(), c.nchunk);
     kskip_cg_gram(n, k, r.data(), p.data(), Ar.data(), Ap.data(), delta.data(), eta.data(), zeta.data());
     eta[2 * k + 1] = g_kskip_oob;
+    if (g_kskip_trace_on) {
+      for (i64 j = 0; j < n; ++j) g_kskip_trace.push_back((double)r[j]);
+      for (i64 j = 0; j < n; ++j) g_kskip_trace.push_back((double)p[j]);
+    }
     for (long il = nloop; il <= nloop + k; ++il) {
       double alpha = gamma / zeta[0];
       double beta = alpha * zeta[1] / zeta[0] - 1.0;
+      if (g_kskip_trace_on) {
+        g_kskip_trace.push_back(alpha);
+        g_kskip_trace.push_back(beta);
+      }
       if (c.fix == 1) {
         gamma = beta * gamma;
       } else if (c.fix == 2) {
@@ -795,6 +808,16 @@ Cfg inner_cfg(const ks_params* p) {
 extern "C" {
 
 void or_set_kskip_oob(double v) { g_kskip_oob = v; }
+// k-skip CG trace: on != 0 clears and enables it, 0 disables it; the size and
+// contents are read with or_kskip_trace_size / or_kskip_trace_copy.
+void or_kskip_trace(int32_t on) {
+  g_kskip_trace_on = on != 0;
+  g_kskip_trace.clear();
+}
+int64_t or_kskip_trace_size(void) { return (int64_t)g_kskip_trace.size(); }
+void or_kskip_trace_copy(double* out) {
+  std::memcpy(out, g_kskip_trace.data(), g_kskip_trace.size() * sizeof(double));
+}
 
 void or_spmv(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* x, double* y) {
   Csr<double> A; A.n = n; A.ptr = ptr; A.col = col; A.val = val;

@@ -57,6 +57,9 @@ def lib():
                                  _f64p, _f64p, C.POINTER(KsResult)]
         L.or_adjoint.restype = C.c_int32
         L.or_set_kskip_oob.argtypes = [C.c_double]
+        L.or_kskip_trace.argtypes = [C.c_int32]
+        L.or_kskip_trace_size.restype = C.c_int64
+        L.or_kskip_trace_copy.argtypes = [_f64p]
         L.or_max_threads.restype = C.c_int32
         _lib = L
     return _lib
@@ -153,6 +156,27 @@ def check(ptr, col, val, b, x):
 
 def set_kskip_oob(v):
     lib().or_set_kskip_oob(float(v))
+
+
+def kskip_trace(ptr, col, val, b, k, **kw):
+    """Run k-skip CG with the oracle's trace on; returns (x, result, blocks)
+    where blocks is a list of (r, p, steps) per block, r and p the block's
+    starting vectors and steps a (k+1, 2) array of (alpha, beta)."""
+    n = len(ptr) - 1
+    L = lib()
+    L.or_kskip_trace(1)
+    try:
+        x, r, _ = solve(ptr, col, val, b, "kskipcg", outer_kskip=k, history=False, **kw)
+        buf = np.empty(L.or_kskip_trace_size(), np.float64)
+        if len(buf):
+            L.or_kskip_trace_copy(buf)
+    finally:
+        L.or_kskip_trace(0)
+    stride = 2 * n + 2 * (k + 1)
+    assert len(buf) % stride == 0
+    blocks = [(buf[o:o + n], buf[o + n:o + 2 * n], buf[o + 2 * n:o + stride].reshape(k + 1, 2))
+              for o in range(0, len(buf), stride)]
+    return x, r, blocks
 
 
 def solve(ptr, col, val, b, outer, inner="NO", x0=None, history=True, hist_cap=1 << 20, **kw):

@@ -22,14 +22,24 @@ cuSPARSE), so each oracle function is pinned by what the mathematics fixes:
   bicg                = CG on SPD matrices (same history); spsolve
   gmres               min-residual history = least squares over the Krylov
                       space (brute force); Givens estimate = true residual;
-                      KS_COMPAT_LAST_TERM leaves the history unchanged
-  kskipcg (derived)   = CG history at the block starts (exact arithmetic)
-  kskipcg (as written)invariant to the out-of-range eta read (0, 1e300, NaN);
-                      otherwise parity unpinned (no independent source)
+                      KS_COMPAT_LAST_TERM leaves the history unchanged, and
+                      its x - x0 = y_{m-1} v_{m-1} of a numpy Arnoldi +
+                      lstsq (the textbook x - x0 = V_m y)
+  kskipcg (derived)   = CG history at the block starts (exact arithmetic);
+                      every step of a block = a textbook CG step (numpy)
+                      from the block's (r, p); k = 0 = CG
+  kskipcg (as written)k = 0 identical to the derived form (no update runs);
+                      the first step of every block = a textbook CG step
+                      from the block's (r, p); invariant to the out-of-range
+                      eta read (0, 1e300, NaN).  The as-written delta/eta/
+                      zeta updates feeding steps 2..k+1 of a block: parity
+                      unpinned (no independent source)
   kskipbicg           = BiCG history at the block starts
   cr, gcr             spsolve solution; GCR(restart) = GMRES history
   vpgmres             inner = one CG step (z = c v) reproduces GMRES;
-                      FP32 inner still reaches the FP64 tolerance
+                      FP32 inner still reaches the FP64 tolerance; the
+                      inner GMRES preconditioner output (last Krylov term
+                      with KS_COMPAT_LAST_TERM, else V_m y) from numpy
   vpcg                inner = one CG step reproduces CG (both beta forms)
   adjoint             diagonal closed form; dense solve; central FD
 """
@@ -299,6 +309,162 @@ def test_kskipcg_as_written_oob_invariance(oracle):
     oracle.set_kskip_oob(0.0)
     for x, r, h in outs[1:]:
         assert np.array_equal(h, outs[0][2]) and np.array_equal(x, outs[0][0])
+
+
+def _cg_steps(A, r, p, m):
+    """m textbook CG steps from (r, p): [(alpha, beta)], and the final (r, p)."""
+    out = []
+    for _ in range(m):
+        Ap = A @ p
+        rr = r @ r
+        alpha = rr / (p @ Ap)
+        r = r - alpha * Ap
+        beta = (r @ r) / rr
+        p = r + beta * p
+        out.append((alpha, beta))
+    return np.array(out), r, p
+
+
+def _spd_case():
+    ptr, col, val, b, _ = P.convdiff3d(6, pe=0.0)
+    return ptr, col, val, b, sp.csr_matrix((val, col, ptr), shape=(len(ptr) - 1,) * 2)
+
+
+@pytest.mark.parametrize("compat", [0, _abi.KS_COMPAT_KSKIP_AS_WRITTEN])
+def test_kskipcg_k0_is_cg(oracle, compat):
+    # k = 0: no delta/eta/zeta update runs (the loop bound 2k - 2(i - nloop)
+    # is 0, kskipcg.cpp:283), so the as-written and derived forms are the same
+    # computation, and each block is one CG step with beta = alpha zeta_1 /
+    # zeta_0 - 1 = (r', r') / (r, r)
+    ptr, col, val, b, A = _spd_case()
+    kw = dict(outer_eps=1e-10, outer_maxloop=500)
+    xk, rk, hk = oracle.solve(ptr, col, val, b, "kskipcg", outer_kskip=0, ref_compat=compat, **kw)
+    xd, rd, hd = oracle.solve(ptr, col, val, b, "kskipcg", outer_kskip=0, **kw)
+    assert np.array_equal(xk, xd) and np.array_equal(hk, hd)
+    xc, rc, hc = oracle.solve(ptr, col, val, b, "cg", **kw)
+    assert rk["status"] == rc["status"] == 0 and abs(rk["iters"] - rc["iters"]) <= 1
+    assert np.linalg.norm(xk - xc) <= 1e-10 * np.linalg.norm(xc)
+    assert np.array_equal(hk[:, 0], hc[:len(hk), 0])
+
+
+@pytest.mark.parametrize("compat", [0, _abi.KS_COMPAT_KSKIP_AS_WRITTEN])
+def test_kskipcg_block_start_steps_are_cg_steps(oracle, compat):
+    # the first inner step of every block uses only the Gram products of that
+    # block's (r, p) (kskipcg.cpp:262-271): alpha = (r,r)/(p,Ap) and beta =
+    # alpha (Ap,Ap)/(p,Ap) - 1 (zeta_1 = (p, A^2 p) = (Ap, Ap) for symmetric
+    # A), from inner products numpy forms on the traced vectors - a wrong Gram
+    # index or product fails here, in both forms.  (This beta is the textbook
+    # (r',r')/(r,r) while p stays A-conjugate to the previous direction, which
+    # the derived form keeps and the diverging as-written form loses.)
+    ptr, col, val, b, A = _spd_case()
+    k = 3
+    x, r, blocks = oracle.kskip_trace(ptr, col, val, b, k, outer_eps=1e-10, outer_maxloop=40, ref_compat=compat)
+    assert len(blocks) >= 5
+    for rb, pb, steps in blocks:
+        Ap = A @ pb
+        alpha = (rb @ rb) / (pb @ Ap)
+        beta = alpha * (Ap @ Ap) / (pb @ Ap) - 1.0
+        assert abs(steps[0, 0] - alpha) <= 1e-10 * abs(alpha)
+        assert abs(steps[0, 1] - beta) <= 1e-9 * max(1.0, abs(beta))
+        if not compat:
+            ref, _, _ = _cg_steps(A, rb, pb, 1)
+            assert abs(steps[0, 1] - ref[0, 1]) <= 1e-8 * max(1.0, abs(ref[0, 1]))
+
+
+def test_kskipcg_derived_every_step_is_cg(oracle):
+    # derived recurrences: all k+1 steps of a block are textbook CG steps from
+    # the block's (r, p), and the next block starts from the r, p they reach
+    ptr, col, val, b, A = _spd_case()
+    k = 3
+    x, r, blocks = oracle.kskip_trace(ptr, col, val, b, k, outer_eps=1e-10, outer_maxloop=40)
+    for i, (rb, pb, steps) in enumerate(blocks):
+        ref, r1, p1 = _cg_steps(A, rb, pb, k + 1)
+        assert np.allclose(steps, ref, rtol=1e-7, atol=1e-12), (i, steps, ref)
+        if i + 1 < len(blocks):
+            rn, pn, _ = blocks[i + 1]
+            assert np.linalg.norm(rn - r1) <= 1e-8 * np.linalg.norm(rb)
+            assert np.linalg.norm(pn - p1) <= 1e-8 * np.linalg.norm(pb)
+    # as written, the later steps of a block leave textbook CG: the switch is live
+    _, _, bw = oracle.kskip_trace(ptr, col, val, b, k, outer_eps=1e-10, outer_maxloop=8,
+                                  ref_compat=_abi.KS_COMPAT_KSKIP_AS_WRITTEN)
+    ref, _, _ = _cg_steps(A, bw[0][0], bw[0][1], k + 1)
+    assert np.max(np.abs(bw[0][2][1:, 0] - ref[1:, 0]) / np.abs(ref[1:, 0])) > 1e-6
+
+
+def _arnoldi(A, v, m):
+    """m Arnoldi steps (classical Gram-Schmidt) from v: V (n, m+1), Hbar (m+1, m)."""
+    n = len(v)
+    V = np.zeros((n, m + 1))
+    H = np.zeros((m + 1, m))
+    V[:, 0] = v / np.linalg.norm(v)
+    for k in range(m):
+        w = A @ V[:, k]
+        h = V[:, :k + 1].T @ w
+        w = w - V[:, :k + 1] @ h
+        H[:k + 1, k] = h
+        H[k + 1, k] = np.linalg.norm(w)
+        V[:, k + 1] = w / H[k + 1, k]
+    return V, H
+
+
+def _gmres_parts(A, b, m):
+    """GMRES after m steps from x0 = 0: (V, y) with y the least-squares
+    solution of Hbar y = ||b|| e_1 (numpy lstsq)."""
+    V, H = _arnoldi(A, b, m)
+    e = np.zeros(m + 1)
+    e[0] = np.linalg.norm(b)
+    y = np.linalg.lstsq(H, e, rcond=None)[0]
+    return V, y
+
+
+def _tiny_nonsym(n=14, seed=3):
+    rng = np.random.default_rng(seed)
+    D = rng.standard_normal((n, n)) * (rng.random((n, n)) < 0.4)
+    D += np.diag(np.abs(D).sum(1) + 1.0)
+    ptr, col, val = _csr(D)
+    return ptr, col, val, rng.standard_normal(n), D
+
+
+def test_gmres_last_term_x_is_last_krylov_term(oracle):
+    # the limit exit after m Arnoldi steps (count+1 >= maxloop, gmres.cpp:286)
+    # assembles x with Scalar_ax, which assigns (blas.cpp:211-224): x = x0 +
+    # y_{m-1} v_{m-1} from the m-step least-squares y; the textbook sum is
+    # x0 + V_m y.  Both from an independent numpy Arnoldi + lstsq.
+    ptr, col, val, b, D = _tiny_nonsym()
+    m = 6
+    V, y = _gmres_parts(D, b, m)
+    last, full = y[m - 1] * V[:, m - 1], V[:, :m] @ y
+    kw = dict(outer_eps=0.0, outer_restart=50, outer_maxloop=m + 1)
+    xc, rc, _ = oracle.solve(ptr, col, val, b, "gmres", ref_compat=_abi.KS_COMPAT_LAST_TERM, **kw)
+    xt, rt, _ = oracle.solve(ptr, col, val, b, "gmres", **kw)
+    assert rc["status"] == rt["status"] == 2 and rc["steps"] == m
+    assert np.linalg.norm(xc - last) <= 1e-10 * np.linalg.norm(last)
+    assert np.linalg.norm(xt - full) <= 1e-10 * np.linalg.norm(full)
+    # an off-by-one term or the full sum is far from it
+    assert np.linalg.norm(full - last) > 1e-2 * np.linalg.norm(last)
+    assert np.linalg.norm(y[m - 2] * V[:, m - 2] - last) > 1e-2 * np.linalg.norm(last)
+
+
+@pytest.mark.parametrize("compat", [0, _abi.KS_COMPAT_LAST_TERM])
+def test_vpgmres_preconditioner_output(oracle, compat):
+    # VPGMRES's inner GMRES exits at its limit (-l L: L-1 steps) and returns
+    # z = M(v_0) (vpgmres.cpp:840): with Scalar_ax's assignment only the last
+    # Krylov term y_{m-1} v_{m-1}, else V_m y.  After one outer step x =
+    # y_0 z with y_0 = ||b|| h00 / (h00^2 + h10^2), the one-column least
+    # squares of w = A z against v_0 = b / ||b||.
+    ptr, col, val, b, D = _tiny_nonsym()
+    m = 5
+    v0 = b / np.linalg.norm(b)
+    V, y = _gmres_parts(D, v0, m)
+    z = y[m - 1] * V[:, m - 1] if compat else V[:, :m] @ y
+    w = D @ z
+    h00 = w @ v0
+    h10 = np.linalg.norm(w - h00 * v0)
+    xr = np.linalg.norm(b) * h00 / (h00 ** 2 + h10 ** 2) * z
+    x, r, _ = oracle.solve(ptr, col, val, b, "vpgmres", "gmres", outer_eps=0.0, outer_restart=20, outer_maxloop=2,
+                           inner_maxloop=m + 1, inner_restart=20, inner_eps=0.0, ref_compat=compat)
+    assert r["steps"] == 1 and r["inner_steps"] == m
+    assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
 
 
 def test_kskipbicg_equals_bicg(oracle):
