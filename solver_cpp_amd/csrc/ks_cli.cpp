@@ -54,6 +54,40 @@ void usage(const std::vector<Opt>& opts) {
               << o.longname << (o.takes_value ? " <value>  (default " + o.value + ")" : "") << "\n";
 }
 
+// The time breakdown in the layout of times::showTimeOnCPU(total, true)
+// followed by times::showTimeOnGPU (times.cpp:44-76), which log scrapers read
+// line by line ("Dot", "MV", "H2D time", "Memset", "other", data2csv.sh:
+// 17-31).  The values are the library's per-class kernel times: Dot = the
+// reductions and Gram-Schmidt projections, MV = the SpMVs and the k-skip
+// matrix-powers basis, other = the rest of the solve's wall time (updates,
+// finishers, launch gaps).  The host does no arithmetic, so the CPU block
+// is zero, and nothing crosses PCIe inside the solve, so the transfer lines
+// are zero.  Expects std::fixed on the stream, as the reference leaves it.
+void print_breakdown(const ks_result& r) {
+  const double total = r.seconds;
+  const double dot = 1e-3 * (r.prof_ms[KS_K_REDUCE] + r.prof_ms[KS_K_CGS]);
+  const double mv = 1e-3 * (r.prof_ms[KS_K_SPMV64] + r.prof_ms[KS_K_SPMV32] + r.prof_ms[KS_K_BASIS]);
+  auto line = [&](const char* label, double v) {
+    std::cout << label << std::setprecision(6) << v << ", " << std::setprecision(2)
+              << (total > 0 ? v / total * 100 : 0.0) << "%" << std::endl;
+  };
+  std::cout << "Execution time on CPU" << std::endl;
+  line("\tDot process time = ", 0.0);
+  line("\tMV process time  = ", 0.0);
+  line("\t  cons time    = ", 0.0);
+  line("\t  dis time    = ", 0.0);
+  std::cout << "Execution time on GPU" << std::endl;
+  line("\tDot process time  = ", dot);
+  line("\tMV process time   = ", mv);
+  line("\t  H2D time       = ", 0.0);
+  line("\t  D2H time       = ", 0.0);
+  line("\t  Memset time    = ", 0.0);
+  line("\tother time       = ", total - dot - mv);
+  line("\t  Copy time    = ", 0.0);
+  line("\t  cons time    = ", 0.0);
+  line("\t  dis time    = ", 0.0);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -156,6 +190,7 @@ int main(int argc, char** argv) {
   p.ref_compat = (uint32_t)std::strtoul(get("ref-compat").c_str(), nullptr, 0);
   p.omp_threads = std::atoi(get("Openmp_Thread").c_str());
   p.verbose = bylong["verbose"]->set;
+  p.profile = (1 << KS_K_NCLASS) - 1;  // per-class times for the breakdown
 
   // collection::checkCMD directory walk (solver_collection.cpp:389-502)
   const std::string full = get("CRS_Path") + "/" + get("L1_Dir_Name") + "/" + get("CRS_Dir_Name") + "/" +
@@ -192,12 +227,35 @@ int main(int argc, char** argv) {
       std::cerr << "[!] could not write the binary cache " << cache << std::endl;
   }
   if (!is_dir("./output")) { std::cerr << "File open error" << std::endl; return 1; }
+  // collection::showCMD (solver_collection.cpp:603-651).  Every solve runs on
+  // the GPU, so "CUDA : On" whether or not -c was given; one GPU is used.
+  const char* rule = "-------------";
   std::cout << "=========================================================" << std::endl;
   std::cout << "InputSource : CRS" << (from_cache ? " (binary cache)" : "") << std::endl
             << "FullPath : " << full << std::endl;
   std::cout << "N : " << n << std::endl << "NNZ : " << nnz << std::endl;
+  std::cout << rule << std::endl;
+  std::cout << "OpenMP thread : " << p.omp_threads << std::endl;
+  std::cout << "GPU Num : 1" << std::endl;
+  std::cout << rule << std::endl;
+  std::cout << "VP : " << (inner ? "On" : "Off") << std::endl;
+  std::cout << "CUDA : On" << std::endl;
+  std::cout << "Pinned : Off" << std::endl;
+  std::cout << "MultiGPU : Off" << std::endl;
+  std::cout << rule << std::endl;
   std::cout << "OuterSolver : " << kUpper[outer] << std::endl;
+  std::cout << "OuterMaxLoop : " << p.outer_maxloop << std::endl;
+  std::cout << "OuterEps : " << p.outer_eps << std::endl;
+  std::cout << "OuterRestart : " << p.outer_restart << std::endl;
+  std::cout << "OuterKskip : " << p.outer_kskip << std::endl;
+  std::cout << "OuterFix : " << p.outer_fix << std::endl;
+  std::cout << rule << std::endl;
   std::cout << "InnerSolver : " << (inner ? kUpper[inner] : "NONE") << std::endl;
+  std::cout << "InnerMaxLoop : " << p.inner_maxloop << std::endl;
+  std::cout << "InnerEps : " << p.inner_eps << std::endl;
+  std::cout << "InnerRestart : " << p.inner_restart << std::endl;
+  std::cout << "InnerKskip : " << p.inner_kskip << std::endl;
+  std::cout << "InnerFix : " << p.inner_fix << std::endl;
   std::cout << "=========================================================" << std::endl;
 
   ks_ctx* ctx = nullptr;
@@ -241,12 +299,7 @@ int main(int argc, char** argv) {
   std::cout << "|b-ax|2/|b|2 = " << std::fixed << std::setprecision(1) << res.log10_check << std::endl;
   std::cout << "loop = " << res.iters << std::endl;
   std::cout << "time = " << std::setprecision(6) << res.seconds << std::endl;
-  std::cout << "Execution time on GPU" << std::endl;
-  const char* cls[] = {"SpMV FP64", "SpMV FP32", "Dot", "Gram-Schmidt", "Axpy", "Scalar", "k-skip basis", "Other"};
-  for (int k = 0; k < KS_K_NCLASS; ++k)
-    if (res.prof_launches[k])
-      std::cout << "\t" << cls[k] << " launches = " << res.prof_launches[k] << ", bytes = " << std::scientific
-                << std::setprecision(3) << res.prof_bytes[k] << std::fixed << std::endl;
+  print_breakdown(res);
   if (res.status == KS_OK) std::cout << "converge" << std::endl;
   else if (res.status == KS_NOT_CONVERGED) std::cout << "not converge" << std::endl;
   else std::cerr << "outer result -> " << res.status << " (" << ks_status_string(res.status) << ")" << std::endl;

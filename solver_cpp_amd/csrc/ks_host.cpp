@@ -381,6 +381,17 @@ bool parse_tokens(const char* b, const char* e, int64_t ntok, Fn fn) {
   return true;
 }
 
+// One column index or row pointer: a decimal integer in [0, INT32_MAX].  A
+// value outside that range is a malformed file, not a different matrix, so
+// it is rejected rather than narrowed.
+bool parse_index(const char* p, char** end, int32_t* out) {
+  errno = 0;
+  const long long v = std::strtoll(p, end, 10);
+  if (errno == ERANGE || v < 0 || v > INT32_MAX) return false;
+  *out = (int32_t)v;
+  return true;
+}
+
 // the text after the three header integers
 const char* after_header(const std::string& buf) {
   const char* p = buf.c_str();
@@ -494,8 +505,9 @@ ks_status ks_crs_read(const char* dir, int32_t* ptr, int32_t* col, double* val, 
   const char* e = buf.c_str() + buf.size();
   bool ok = parse_tokens(after_header(buf), e, 2 * nnz, [&](int64_t i, const char* p, char** end) {
     if (i % 2 == 0) {
-      const long c = std::strtol(p, end, 10);
-      if (col) col[i / 2] = (int32_t)c;
+      int32_t c;
+      if (!parse_index(p, end, &c)) return false;
+      if (col) col[i / 2] = c;
     } else {
       const double v = std::strtod(p, end);
       if (val) val[i / 2] = v;
@@ -506,8 +518,9 @@ ks_status ks_crs_read(const char* dir, int32_t* ptr, int32_t* col, double* val, 
   if (!read_all(d + "/Ptr.txt", buf)) return KS_ERR_IO;
   e = buf.c_str() + buf.size();
   ok = parse_tokens(after_header(buf), e, n + 1, [&](int64_t i, const char* p, char** end) {
-    const long c = std::strtol(p, end, 10);
-    if (ptr) ptr[i] = (int32_t)c;
+    int32_t c;
+    if (!parse_index(p, end, &c)) return false;
+    if (ptr) ptr[i] = c;
     return true;
   });
   if (!ok) return KS_ERR_IO;

@@ -227,6 +227,26 @@ def test_crs_parallel_parse_any_whitespace(tmp_path):
         gen.read_crs(d)
 
 
+@pytest.mark.parametrize("which,token", [("col", "4294967301"), ("col", "-1"), ("ptr", "2147483648"),
+                                         ("col", "99999999999999999999999")])
+def test_crs_index_out_of_int32_range(tmp_path, which, token):
+    # an index that does not fit the int32 CSR arrays is a malformed file:
+    # 4294967301 must not wrap to 5 and pass the column-range check as a
+    # different valid matrix
+    ptr, col, val = gen.stencil(2, 6, 5)
+    n = len(ptr) - 1
+    d = str(tmp_path / "M")
+    gen.write_crs(d, ptr, col, val, gen.vector(3, n, "uniform"))
+    fn = os.path.join(d, "ColVal.txt" if which == "col" else "Ptr.txt")
+    lines = open(fn).read().split("\n")
+    f = lines[3].split()
+    f[0] = token
+    lines[3] = " ".join(f)
+    open(fn, "w").write("\n".join(lines))
+    with pytest.raises(RuntimeError):
+        gen.read_crs(d)
+
+
 def test_crs_binary_sidecar(tmp_path):
     ptr, col, val = gen.stencil(3, 9, 8, 7, pe=0.8)
     n = len(ptr) - 1
@@ -283,6 +303,28 @@ def test_cli_flag_errors(tmp_path):
     assert r.returncode != 0
     r = _cli("-h")
     assert r.returncode == 0 and "OuterSolver" in r.stderr
+
+
+def test_cli_header_block_as_reference(tmp_path):
+    # collection::showCMD's block (solver_collection.cpp:603-651) is printed
+    # before the GPU is opened; the reference's log scraper reads its
+    # "CUDA :" and "MultiGPU :" lines (data2csv.sh:6-15)
+    from tests._scrape import seek
+    ptr, col, val = gen.stencil(2, 6, 6, 1, pe=0.0)
+    root = tmp_path / "root"
+    gen.write_crs(str(root / "Matrix" / "CRS" / "P6"), ptr, col, val, gen.vector(1, 36, "uniform"))
+    work = tmp_path / "work"
+    (work / "output").mkdir(parents=True)
+    r = _cli("-S", "vpgmres", "-s", "gmres", "-R", "21", "-l", "11", "-m", "P6", "-p", str(root), cwd=str(work))
+    lines = r.stdout.splitlines()
+    for want in ("N : 36", "NNZ : 156", "VP : On", "CUDA : On", "Pinned : Off", "MultiGPU : Off",
+                 "OuterSolver : VPGMRES", "OuterRestart : 21", "InnerSolver : GMRES", "InnerMaxLoop : 11",
+                 "OpenMP thread : 8"):
+        assert want in lines, want
+    f = seek(r.stdout)
+    assert f["is_gpu"] and not f["is_multigpu"]
+    # nothing in the block may be taken for the fields scraped from the solve
+    assert f["Total"] is None and f["Loop"] is None and f["CPU-Dot"] is None and f["Other"] is None
 
 
 def test_cli_bin_cache_written_then_used(tmp_path):
