@@ -226,6 +226,13 @@ def test_inner_solver_kinds(ctx, oracle):
     for inner in ("cg", "cr", "gcr", "bicg", "gmres"):
         _run(ctx, oracle, ptr, col, val, b, "vpgmres", inner, outer_eps=1e-10, outer_maxloop=200,
              outer_restart=21, inner_maxloop=5, inner_restart=8, xtol=1e-9)
+    # the k-skip inner solvers (FP64): a block of k+1 steps per basis pass,
+    # -l 6 inner iterations with k = 2 (kskipcg.cpp:187-343,
+    # kskipbicg.cpp:207-385 as innerSelect builds them)
+    for inner in ("kskipcg", "kskipbicg"):
+        for outer in ("vpgmres", "vpgcr"):
+            _run(ctx, oracle, ptr, col, val, b, outer, inner, outer_eps=1e-10, outer_maxloop=200,
+                 outer_restart=21, inner_maxloop=6, inner_kskip=2, inner_fix=2, xtol=1e-9)
 
 
 def test_zero_rhs(ctx):
@@ -320,4 +327,17 @@ def test_cli_solver_end_to_end(tmp_path, oracle):
         assert np.array_equal(his[:, 0], ho[:, 0])  # same check indices as the oracle
         assert np.linalg.norm(xv - xo) <= 1e-10 * np.linalg.norm(xo)
         outs.append(xv)
+        # the reference's log scraper (data2csv.sh:17-31) reads the same
+        # numbers back: total time, loop, converge, the Dot / MV / H2D / D2H /
+        # Memset / other fields of the GPU breakdown (times.cpp:60-76)
+        from tests._scrape import seek
+        f = seek(r.stdout)
+        tot = float(re.search(r"^time = ([0-9.]+)$", r.stdout, re.M).group(1))
+        assert f["is_gpu"] and float(f["Total"]) == tot > 0
+        assert f["Loop"] == str(loop) and f["Converge"] == "converge"
+        assert float(f["CPU-Dot"]) == 0.0 and float(f["CPU-MV"]) == 0.0
+        gdot, gmv, oth = float(f["GPU-Dot"]), float(f["GPU-MV"]), float(f["Other"])
+        assert gdot > 0 and gmv > 0 and oth >= 0
+        assert abs(gdot + gmv + oth - tot) <= 3e-6
+        assert float(f["GPU-H2D"]) == float(f["GPU-D2H"]) == float(f["GPU-Memset"]) == 0.0
     assert np.array_equal(outs[0], outs[1])
