@@ -10,6 +10,15 @@ Backward (hand-written adjoint, no reference counterpart): for a loss L(x),
                                transposed CSR otherwise)
   dL/db      = lambda
   dL/dA_ij   = -lambda_i x_j   on the sparsity pattern (SDDMM).
+
+Row scaling (SparseSystem(row_scale=True)): the solves run on the
+equilibrated system D^-1 A x = D^-1 b, D = diag(A), whose rows all have unit
+diagonal (for a matrix whose row scales spread over decades, where restarted
+Krylov methods on A itself stagnate).  x is the same; the adjoint solves
+(D^-1 A)^T mu = dL/dx on the explicit transpose and maps back: lambda =
+D^-1 mu, dL/dA_ij = -mu_i x_j / d_i.  D only changes how the systems are
+solved, not the function x(A, b), so the gradient is that of the unscaled
+system.
 """
 import numpy as np
 import torch
@@ -17,17 +26,51 @@ import torch
 from . import api
 
 
+class RowScale:
+    """Row index of every entry and the slot of every row's diagonal entry of
+    a CSR pattern (on the device), for D = diag(A) row scaling."""
+
+    def __init__(self, ptr, col, device="cuda"):
+        n = len(ptr) - 1
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ptr))
+        on_diag = np.flatnonzero(col == rows)
+        dslot = np.full(n, -1, np.int64)
+        dslot[rows[on_diag]] = on_diag
+        if (dslot < 0).any():
+            raise ValueError(f"row scaling needs a diagonal entry in every row "
+                             f"(row {int(np.flatnonzero(dslot < 0)[0])} has none)")
+        self.rows = torch.from_numpy(rows).to(device)
+        self.dslot = torch.from_numpy(dslot).to(device)
+
+    def diag(self, vals):
+        d = vals.detach()[self.dslot]
+        if not bool(torch.all(torch.isfinite(d) & (d != 0))):
+            raise ValueError("row scaling needs a finite nonzero diagonal in every row")
+        return d
+
+    def scaled(self, vals, d):
+        """values of D^-1 A"""
+        return (vals.detach() / d[self.rows]).contiguous()
+
+
 class SparseSystem:
-    """A CSR pattern in HBM whose values come from a tensor at each call."""
+    """A CSR pattern in HBM whose values come from a tensor at each call.
+
+    row_scale=True solves the row-equilibrated system D^-1 A x = D^-1 b
+    (D = diag(A); every row needs a nonzero diagonal entry), see the module
+    docstring."""
 
     def __init__(self, ctx, ptr, col, n=None, spd=False, solver="gmres", inner="NO", params=None,
-                 adjoint_solver=None, adjoint_inner=None, adjoint_params=None, fp32=False):
+                 adjoint_solver=None, adjoint_inner=None, adjoint_params=None, fp32=False, row_scale=False):
         self.ctx = ctx
         ptr = np.ascontiguousarray(ptr, np.int32)
         col = np.ascontiguousarray(col, np.int32)
         self.n = len(ptr) - 1 if n is None else n
         self.nnz = len(col)
         self.ptr, self.col = ptr, col
+        if row_scale and spd:
+            raise ValueError("row scaling makes the matrix nonsymmetric: use it with spd=False")
+        self.scale = RowScale(ptr, col) if row_scale else None
         self.A = api.Matrix(ctx, ptr, col, np.zeros(self.nnz), transpose=not spd, fp32=fp32, spd=spd)
         self.solver, self.inner = solver, inner
         self.params = dict(params or {})
@@ -39,9 +82,16 @@ class SparseSystem:
         self.last_backward = None
 
     def _set(self, vals):
-        # always refresh: the tensor may have been modified in place
-        self.A.update_values(vals.detach().contiguous())
+        """Load the values (of D^-1 A with row scaling); returns D's diagonal
+        (None without row scaling).  Always refreshes: the tensor may have
+        been modified in place."""
         self._vals = vals
+        if self.scale is None:
+            self.A.update_values(vals.detach().contiguous())
+            return None
+        d = self.scale.diag(vals)
+        self.A.update_values(self.scale.scaled(vals, d))
+        return d
 
     def solve(self, vals, b):
         return KrylovSolve.apply(vals, b, self)
@@ -52,9 +102,9 @@ class KrylovSolve(torch.autograd.Function):
     def forward(ctx, vals, b, system):
         if not (vals.is_cuda and b.is_cuda):
             raise TypeError("KrylovSolve expects device tensors")
-        system._set(vals)
-        x, res, _ = api.solve(system.ctx, system.A, b.detach().contiguous(), system.solver, system.inner,
-                              history=False, **system.params)
+        d = system._set(vals)
+        rhs = b.detach().contiguous() if d is None else (b.detach() / d).contiguous()
+        x, res, _ = api.solve(system.ctx, system.A, rhs, system.solver, system.inner, history=False, **system.params)
         system.last_forward = res
         if res["status"] != 0:
             raise RuntimeError(f"forward solve did not converge: {res}")
@@ -66,12 +116,16 @@ class KrylovSolve(torch.autograd.Function):
     def backward(ctx, gx):
         vals, x = ctx.saved_tensors
         system = ctx.system
-        system._set(vals)
+        d = system._set(vals)
         gb, gv, res = api.adjoint(system.ctx, system.A, x, gx.contiguous(), system.adjoint_solver,
                                   system.adjoint_inner, **system.adjoint_params)
         system.last_backward = res
         if res["status"] != 0:
             raise RuntimeError(f"adjoint solve did not converge: {res}")
+        if d is not None:
+            # (D^-1 A)^T mu = dL/dx gave mu and -mu_i x_j: lambda = D^-1 mu
+            gb = gb / d
+            gv = gv / d[system.scale.rows]
         return gv, gb, None
 
 
@@ -89,10 +143,13 @@ class CentralFD:
     on the device.
     """
 
-    def __init__(self, ctx, ptr, col, vals, b, x_target, eps, solver, inner="NO", params=None, spd=False):
+    def __init__(self, ctx, ptr, col, vals, b, x_target, eps, solver, inner="NO", params=None, spd=False,
+                 row_scale=False):
         self.ctx, self.b, self.xt, self.eps = ctx, b, x_target, eps
         self.solver, self.inner, self.params = solver, inner, dict(params or {})
         self.vals = torch.as_tensor(vals).detach().double().cuda().contiguous()
+        # row_scale: each solve runs on its matrix's D^-1 A (see SparseSystem)
+        self.scale = RowScale(ptr, col) if row_scale else None
         fp32 = self.params.get("inner_precision", 64) == 32
         v = self.vals.cpu().numpy()
         self.Am = api.Matrix(ctx, ptr, col, v, transpose=not spd, spd=spd, fp32=fp32)
@@ -101,19 +158,30 @@ class CentralFD:
 
     def __call__(self, dvals, naive=False):
         d = torch.as_tensor(dvals).detach().double().cuda().contiguous()
-        self.Am.update_values((self.vals - self.eps * d).contiguous())
-        self.Ap.update_values((self.vals + self.eps * d).contiguous())
+        vm = (self.vals - self.eps * d).contiguous()
+        vp = (self.vals + self.eps * d).contiguous()
         self.D.update_values(d)
         kw = dict(history=False, **self.params)
-        xm, rm, _ = api.solve(self.ctx, self.Am, self.b, self.solver, self.inner, **kw)
-        u, ru, _ = api.solve(self.ctx, self.Ap, self.D.spmv(xm), self.solver, self.inner, **kw)
+        if self.scale is None:
+            self.Am.update_values(vm)
+            self.Ap.update_values(vp)
+            xm, rm, _ = api.solve(self.ctx, self.Am, self.b, self.solver, self.inner, **kw)
+            u, ru, _ = api.solve(self.ctx, self.Ap, self.D.spmv(xm), self.solver, self.inner, **kw)
+        else:
+            dm, dp = self.scale.diag(vm), self.scale.diag(vp)
+            self.Am.update_values(self.scale.scaled(vm, dm))
+            self.Ap.update_values(self.scale.scaled(vp, dp))
+            xm, rm, _ = api.solve(self.ctx, self.Am, (self.b / dm).contiguous(), self.solver, self.inner, **kw)
+            u, ru, _ = api.solve(self.ctx, self.Ap, (self.D.spmv(xm) / dp).contiguous(), self.solver, self.inner,
+                                 **kw)
         for r in (rm, ru):
             if r["status"] != 0:
                 raise RuntimeError(f"FD solve did not converge: {r}")
         fd = float(-(u @ (xm - self.xt)) + self.eps * (u @ u))
         if not naive:
             return fd
-        xp, rp, _ = api.solve(self.ctx, self.Ap, self.b, self.solver, self.inner, **kw)
+        bp = self.b if self.scale is None else (self.b / dp).contiguous()
+        xp, rp, _ = api.solve(self.ctx, self.Ap, bp, self.solver, self.inner, **kw)
         nv = float((0.5 * ((xp - self.xt) ** 2).sum() - 0.5 * ((xm - self.xt) ** 2).sum()) / (2 * self.eps))
         return fd, nv
 
@@ -123,10 +191,10 @@ class CentralFD:
 
 
 def central_fd(ctx, ptr, col, vals, dvals, b, x_target, eps, solver, inner="NO", params=None, spd=False,
-               naive=True):
+               naive=True, row_scale=False):
     """One CentralFD evaluation: (fd, naive_fd) where naive_fd subtracts the
     two losses directly (a third solve); with naive=False returns fd only."""
-    f = CentralFD(ctx, ptr, col, vals, b, x_target, eps, solver, inner, params, spd)
+    f = CentralFD(ctx, ptr, col, vals, b, x_target, eps, solver, inner, params, spd, row_scale)
     try:
         return f(dvals, naive=naive)
     finally:

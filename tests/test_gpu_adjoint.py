@@ -88,3 +88,42 @@ def test_nonsymmetric_vpgmres_forward_gmres_adjoint(ctx, case):
                                   params=fwd)
         ad = float(vals.grad @ dv)
         assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad, naive)
+
+
+def test_zipf_heavy_rows_row_scaled(ctx):
+    # Zipf rows up to 2048 entries: diagonals spread over decades and
+    # restarted GMRES on A stagnates; on D^-1 A (SparseSystem(row_scale=True))
+    # mixed VPGMRES and the adjoint GMRES converge, and the gradient is that of
+    # the unscaled system: the dense FP64 gradient and central FD
+    import torch
+    from solver_cpp_amd import autograd as AG
+    from solver_cpp_amd import gen
+    ptr, col, val = gen.zipf(3000, 1.5, 2048, (51, 52, 53))
+    n = len(ptr) - 1
+    assert np.diff(ptr).max() > 1000
+    b = torch.from_numpy(gen.vector(50, n, "uniform")).cuda()
+    xt = torch.from_numpy(gen.vector(5, n, "normal")).cuda()
+    fwd = dict(outer_eps=1e-14, outer_maxloop=500, outer_restart=21, inner_maxloop=11, inner_restart=12,
+               inner_precision=32)
+    sysm = AG.SparseSystem(ctx, ptr, col, spd=False, solver="vpgmres", inner="gmres", fp32=True, params=fwd,
+                           adjoint_solver="gmres", adjoint_params=dict(outer_eps=1e-14, outer_maxloop=2000,
+                                                                       outer_restart=21), row_scale=True)
+    vals = torch.from_numpy(val).cuda().requires_grad_(True)
+    bb = b.clone().requires_grad_(True)
+    x = sysm.solve(vals, bb)
+    L = 0.5 * ((x - xt) ** 2).sum()
+    L.backward()
+    assert sysm.last_forward["status"] == 0 and sysm.last_backward["status"] == 0
+    assert sysm.last_forward["iters"] < 50 and sysm.last_backward["iters"] < 200
+    lam, gv = _dense_grad(ptr, col, val, x.detach().cpu().numpy(), (x - xt).detach().cpu().numpy())
+    assert np.linalg.norm(bb.grad.cpu().numpy() - lam) <= 1e-9 * np.linalg.norm(lam)
+    assert np.linalg.norm(vals.grad.cpu().numpy() - gv) <= 1e-9 * np.linalg.norm(gv)
+    rng = np.random.default_rng(54)
+    fdx = AG.CentralFD(ctx, ptr, col, vals.detach(), b, xt, 1e-7, "vpgmres", "gmres", params=fwd, row_scale=True)
+    for _ in range(6):
+        dv = torch.from_numpy(rng.standard_normal(len(val))).cuda()
+        fd, naive = fdx(dv, naive=True)
+        ad = float(vals.grad @ dv)
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad, naive)
+    fdx.close()
+

@@ -8,6 +8,13 @@ that hold at any size (the oracle's own runs are kept to small sizes).
   bit-identical run to run; and x lies within kappa(A) * 1e-10 of the FP64
   GMRES solution, kappa(A) = lambda_max / lambda_min of the 7-point stencil
   in closed form.
+- The adjoint of the 4,000,000-row Zipf(1.5) matrix (rows capped at 4096
+  entries, 196 M nonzeros): forward mixed VPGMRES(20) with ten FP32 GMRES
+  inner steps, backward GMRES(20) on the explicit transpose, both on the
+  row-equilibrated system D^-1 A (D = diag(A)), to 1e-14.  Checked: the
+  unscaled true residuals of both solves, 64 central finite differences
+  along relative value directions, and the oracle's FP64 adjoint solve of
+  the same system on the host.
 - SpMV (cudaFunction.cu:127-156, blas.cpp:43-129) and the explicit A^T
   (cudaFunction.cu:896-931) at the full sizes of the 160^3 Poisson, 192^3
   convection-diffusion and 4,000,000-row Zipf matrices (the last on column
@@ -16,6 +23,7 @@ that hold at any size (the oracle's own runs are kept to small sizes).
   entry for entry to scipy's stable CSC -> CSR conversion.
 """
 import math
+import time
 
 import numpy as np
 import pytest
@@ -105,3 +113,83 @@ def test_spmv_full_size(ctx, cfg):
     b32 = 4 * m * EPS32 * P.row_abs_bound(ptr, col, v32.astype(np.float64), x32.astype(np.float64)) + 1e-30
     assert np.all(np.abs(y32.astype(np.float64) - S32 @ x32.astype(np.float64)) <= b32)
     A.close()
+
+
+def test_zipf_adjoint_full_size(ctx, oracle):
+    import torch
+    from solver_cpp_amd import api, gen
+    from solver_cpp_amd import autograd as AG
+    t0 = time.perf_counter()
+    Pb = gen.problem("zipf")
+    ptr, col, val = Pb["ptr"], Pb["col"], Pb["val"]
+    n, nnz = len(ptr) - 1, len(col)
+    assert n == 4_000_000 and np.diff(ptr).max() == 4096
+    print(f"[zipf-adjoint] generated n={n} nnz={nnz} in {time.perf_counter() - t0:.1f} s", flush=True)
+    b = torch.from_numpy(Pb["b"]).cuda()
+    xt = torch.from_numpy(Pb["x_target"]).cuda()
+    fwd = dict(outer_eps=1e-14, outer_maxloop=400, outer_restart=21, inner_maxloop=11, inner_restart=12,
+               inner_eps=0.0, inner_precision=32)
+    adj = dict(outer_eps=1e-14, outer_maxloop=2000, outer_restart=21)
+    sysm = AG.SparseSystem(ctx, ptr, col, spd=False, solver="vpgmres", inner="gmres", fp32=True, params=fwd,
+                           adjoint_solver="gmres", adjoint_params=adj, row_scale=True)
+    vals = torch.from_numpy(val).cuda().requires_grad_(True)
+    bb = b.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    x = sysm.solve(vals, bb)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    L = 0.5 * ((x - xt) ** 2).sum()
+    L.backward()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    rf, rb = sysm.last_forward, sysm.last_backward
+    print(f"[zipf-adjoint] forward {t2 - t1:.3f} s ({rf['iters']} it), backward {t3 - t2:.3f} s "
+          f"({rb['iters']} it)", flush=True)
+    assert rf["status"] == 0 and rb["status"] == 0
+    # unscaled true residuals with the library's FP64 SpMV of A and A^T
+    A = api.Matrix(ctx, ptr, col, val, transpose=True)
+    g = (x - xt).detach()
+    lam = bb.grad
+    fr = float(torch.linalg.norm(b - A.spmv(x.detach())) / torch.linalg.norm(b))
+    br = float(torch.linalg.norm(A.spmv(lam, trans=True) - g) / torch.linalg.norm(g))
+    print(f"[zipf-adjoint] unscaled relres forward {fr:.2e}, adjoint {br:.2e}", flush=True)
+    assert fr <= 1e-11 and br <= 1e-11
+    A.close()
+    # the oracle's FP64 GMRES(20) on (D^-1 A)^T from the same x and g
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    dg = np.zeros(n)
+    on = col == rows
+    dg[rows[on]] = val[on]
+    gh = g.cpu().numpy()
+    mu, _, ro = oracle.adjoint(ptr, col, val / dg[rows], x.detach().cpu().numpy(), gh, "gmres",
+                               omp_threads=min(oracle.max_threads(), 16), **adj)
+    lo = mu / dg
+    assert ro["status"] == 0
+    lg = lam.cpu().numpy()
+    rel = np.linalg.norm(lg - lo) / np.linalg.norm(lo)
+    print(f"[zipf-adjoint] grad_b vs oracle {rel:.2e} ({ro['iters']} oracle it)", flush=True)
+    assert rel <= 1e-9
+    # 64 central finite differences (eps 1e-7) along seeded relative value
+    # directions d_k = a_k z_k, z ~ N(0,1): every entry is perturbed in
+    # proportion to its own size.  (Absolute N(0,1) perturbations of 1e-7 are
+    # comparable to the smallest diagonals here, where a row's only
+    # off-diagonal entry is tiny, and the difference quotient's O(eps^2)
+    # term then swamps it.)
+    gen_d = torch.Generator(device="cuda")
+    gen_d.manual_seed(54)
+    v0 = vals.detach()
+    fdx = AG.CentralFD(ctx, ptr, col, v0, b, xt, 1e-7, "vpgmres", "gmres", params=fwd, row_scale=True)
+    worst = 0.0
+    t4 = time.perf_counter()
+    for i in range(64):
+        dv = v0 * torch.randn(nnz, dtype=torch.float64, device="cuda", generator=gen_d)
+        fd = fdx(dv)
+        ad = float(vals.grad @ dv)
+        worst = max(worst, abs(fd - ad) / abs(ad))
+    fdx.close()
+    print(f"[zipf-adjoint] 64 FD directions in {time.perf_counter() - t4:.1f} s, max rel err {worst:.2e}",
+          flush=True)
+    assert worst <= 1e-6
+    sysm.A.close()
+
