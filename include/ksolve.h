@@ -229,6 +229,21 @@ ks_status ks_sddmm(ks_mat* m, double alpha, const double* u, const double* v,
 ks_status ks_solve(ks_ctx* ctx, ks_mat* m, const ks_params* p, const double* b,
                    double* x, double* hist, int64_t hist_cap, ks_result* res);
 
+/* No-allocation variant of ks_solve.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a
+ * call (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It runs plain FP64 CG (p->outer "cg", p->inner "NO")
+ * with the same kernels and results as ks_solve, on device pointers only
+ * (KS_PTR_DEVICE), with no residual history and no profiling, for matrices
+ * of at most 1,048,576 rows (KS_ERR_BAD_INPUT beyond that).  Its vectors and
+ * control state live in a fixed device array of about 24 MB that the library
+ * holds on each GPU it runs on until the process ends; it calls no hipMalloc
+ * or hipFree.  Calls on one GPU share that array: the library serialises
+ * them within a process (one at a time per device), and each call keeps the
+ * array's contents until the next. */
+ks_status ks_solve_noalloc(ks_ctx* ctx, ks_mat* m, const ks_params* p, const double* b,
+                           double* x, ks_result* res);
+
 /* ---- adjoint ------------------------------------------------------------
  * No reference counterpart.  Given the forward solution x of A x = b and
  * g = dL/dx, solves A^T lambda = g with the solver named in p (cg for SPD,

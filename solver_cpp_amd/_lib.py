@@ -43,6 +43,7 @@ _SIGS = {
     "ks_kskip_gram": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32]),
     "ks_sddmm": (_i32, [_vp, _dbl, _vp, _vp, _vp, _u32]),
     "ks_solve": (_i32, [_vp, _vp, C.POINTER(KsParams), _vp, _vp, _vp, _i64, C.POINTER(KsResult)]),
+    "ks_solve_noalloc": (_i32, [_vp, _vp, C.POINTER(KsParams), _vp, _vp, C.POINTER(KsResult)]),
     "ks_adjoint": (_i32, [_vp, _vp, C.POINTER(KsParams), _vp, _vp, _vp, _vp, C.POINTER(KsResult)]),
     "ks_gen_stencil": (_i32, [_i32, _i32, _i32, _i32, _dbl, _vp, C.POINTER(_i64), C.POINTER(_i64), _vp, _vp, _vp]),
     "ks_gen_aniso2d": (_i32, [_i32, _i32, _u64, C.POINTER(_i64), C.POINTER(_i64), _vp, _vp, _vp, _vp]),

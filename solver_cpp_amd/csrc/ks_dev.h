@@ -152,6 +152,7 @@ struct ks_ctx {
   int num_cus = 256;
   ks::Prof prof;
   std::vector<hipEvent_t> evpool;
+  double* scratch = nullptr;             // device address of the no-allocation solve's scratch array
   // resident workgroups per CU of each kernel (and dynamic LDS size) launched
   // on this context's device
   std::map<std::pair<const void*, size_t>, int> occ;

@@ -2,8 +2,8 @@
 // (upload, FP32 copy, device transpose), BLAS-layer operators, ks_solve and
 // ks_adjoint.  All work is ordered on the context's HIP stream.
 #include <chrono>
-#include <deque>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 #include "ks_levels.h"
@@ -530,7 +530,10 @@ void run_outer(ks_ctx* c, Level& L, const double* b, double* x, int check_every)
     B = (int64_t)std::ceil(4e-4 / est);
     B = std::min<int64_t>(std::max<int64_t>(B, 1), 4096);
   }
-  std::deque<hipEvent_t> q;
+  // the events of the last two batches (the pool is pre-filled by
+  // ks_ctx_create, so a solve creates none)
+  hipEvent_t q[2] = {nullptr, nullptr};
+  int nq = 0;
   int64_t u = 0;
   bool stopped = false;
   while (u < U && !stopped) {
@@ -538,22 +541,28 @@ void run_outer(ks_ctx* c, Level& L, const double* b, double* x, int check_every)
     for (int64_t i = 0; i < nb; ++i) L.enqueue_unit(u++);
     hipEvent_t ev = ev_get(c);
     KS_HIPCK(hipEventRecord(ev, c->stream));
-    q.push_back(ev);
-    if (q.size() >= 2) {
-      KS_HIPCK(hipEventSynchronize(q.front()));
-      c->evpool.push_back(q.front());
-      q.pop_front();
+    q[nq++] = ev;
+    if (nq == 2) {
+      KS_HIPCK(hipEventSynchronize(q[0]));
+      c->evpool.push_back(q[0]);
+      q[0] = q[1];
+      nq = 1;
       if (hf[0]) stopped = true;
     }
   }
   L.enqueue_finish();
   KS_HIPCK(hipStreamSynchronize(c->stream));
-  for (auto e : q) c->evpool.push_back(e);
+  for (int i = 0; i < nq; ++i) c->evpool.push_back(q[i]);
 }
 
-// ||b - A x||_2 on the device (FP64), with a scratch residual.
-double resid_norm(ks_ctx* c, ks_mat* m, bool trans, const double* b, const double* x) {
-  DBuf<double> r(m->n);
+// ||b - A x||_2 on the device (FP64); the residual goes to rbuf (n doubles)
+// when given, else to a temporary.
+double resid_norm(ks_ctx* c, ks_mat* m, bool trans, const double* b, const double* x, double* rbuf = nullptr) {
+  DBuf<double> r;
+  if (rbuf)
+    r.borrow(rbuf, m->n);
+  else
+    r.alloc(m->n);
   auto A = m->view<double>(trans && m->has_t);
   launch_spmv<double, 1>(c, A, x, EpiResid<double, FinStore>{b, r.p, FinStore{c->scal.p + 1, 1}}, Gate{}, 8.0 * m->n);
   double v = 0;
@@ -580,14 +589,20 @@ int64_t total_inner_steps(Level& L) {
   return h.tsteps;
 }
 
-void solve_impl(ks_ctx* c, ks_mat* m, const ks_params* p, bool trans, const double* b, double* x, double* hist,
-                int64_t hist_cap, ks_result* res, double* x0norm_out) {
-  auto L = make_solver(c, m, p, trans, hist ? hist_cap : 0);
+// Run a constructed solve level and fill the result.  prof_mask selects the
+// kernel classes timed with events; rbuf (optional, n doubles of device
+// memory the solve no longer needs at the end) holds the residuals of the
+// true-residual checks.
+void run_level(ks_ctx* c, ks_mat* m, Level& Lv, const ks_params* p, unsigned prof_mask, bool trans,
+               const double* b, double* x, double* hist, int64_t hist_cap, ks_result* res, double* x0norm_out,
+               double* rbuf) {
+  Level* L = &Lv;
   L->x0given = (p->flags & KS_X0_GIVEN) != 0;
   double r0 = 0;
-  if (L->x0given) r0 = resid_norm(c, m, trans, b, x);
-  KS_HIPCK(hipMemsetAsync(c->stats.p, 0, sizeof(unsigned long long) * 2 * KS_K_NCLASS, c->stream));
-  c->prof.mask = p->profile > 0 ? (unsigned)p->profile : 0u;
+  if (L->x0given) r0 = resid_norm(c, m, trans, b, x, rbuf);
+  k_fill<unsigned long long><<<1, kBlock, 0, c->stream>>>(2 * KS_K_NCLASS, c->stats.p, 0ull);
+  KS_HIPCK(hipGetLastError());
+  c->prof.mask = prof_mask;
   KS_HIPCK(hipStreamSynchronize(c->stream));
   const auto t0 = std::chrono::steady_clock::now();
   run_outer(c, *L, b, x, p->check_every);
@@ -616,7 +631,7 @@ void solve_impl(ks_ctx* c, ks_mat* m, const ks_params* p, bool trans, const doub
       tot += (double)st[KS_K_NCLASS + k];
     }
     res->bytes = tot;
-    const double rn = resid_norm(c, m, trans, b, x);
+    const double rn = resid_norm(c, m, trans, b, x, rbuf);
     const double bn = norm_dev(c, m->n, b);
     res->relres_true = bn > 0 ? rn / bn : rn;
     const double den = L->x0given ? r0 : bn;
@@ -628,6 +643,54 @@ void solve_impl(ks_ctx* c, ks_mat* m, const ks_params* p, bool trans, const doub
     }
   }
   if (x0norm_out) *x0norm_out = r0;
+}
+
+void solve_impl(ks_ctx* c, ks_mat* m, const ks_params* p, bool trans, const double* b, double* x, double* hist,
+                int64_t hist_cap, ks_result* res, double* x0norm_out) {
+  auto L = make_solver(c, m, p, trans, hist ? hist_cap : 0);
+  run_level(c, m, *L, p, p->profile > 0 ? (unsigned)p->profile : 0u, trans, b, x, hist, hist_cap, res, x0norm_out,
+            nullptr);
+}
+
+// ------------------------------------------------ no-allocation CG solve ---
+// ks_solve_noalloc keeps every piece of per-call device state of a plain FP64
+// CG solve (r, p, q, the small arrays and the control block) in `scratch`, a
+// file-scope device array of kNoAllocMaxRows-row capacity (3 * 2^20 doubles
+// of vectors + the control state, about 24 MB held by each GPU the library
+// runs on until the process ends).  In host code `scratch` names a host-side
+// stand-in, never the device memory: its device address on each GPU comes
+// from hipGetSymbolAddress, taken once per device with that device current
+// (scratch_address, from ks_ctx_create), and only that address is passed to
+// kernels.  Calls on one GPU share the array: a per-device lock serialises
+// them within the process, and each call clears what must start at zero.
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 20;
+constexpr int kScratchMaxDevices = 64;
+__device__ __align__(256) double scratch[3 * kNoAllocMaxRows + 1024];
+
+struct ScratchTable {
+  std::mutex mu;                       // guards addr
+  double* addr[kScratchMaxDevices] = {};
+  std::mutex call[kScratchMaxDevices];  // one ks_solve_noalloc at a time per device
+};
+ScratchTable& scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+
+// Device address of `scratch` on `device` (which must be current); nullptr
+// for device numbers beyond the table.
+double* scratch_address(int device) {
+  if (device < 0 || device >= kScratchMaxDevices) return nullptr;
+  ScratchTable& t = scratch_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  if (!t.addr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw Error(KS_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    t.addr[device] = (double*)ptr;
+  }
+  return t.addr[device];
 }
 
 }  // namespace
@@ -668,6 +731,15 @@ ks_status ks_ctx_create(int32_t device, ks_ctx** out) {
     hipDeviceProp_t prop;
     KS_HIPCK(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
+    // the batch events of run_outer (it holds at most three at a time), so
+    // that a solve creates none
+    c->evpool.reserve(16);
+    for (int i = 0; i < 4; ++i) {
+      hipEvent_t e;
+      KS_HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      c->evpool.push_back(e);
+    }
+    c->scratch = scratch_address(device);
     KS_HIPCK(hipStreamSynchronize(c->stream));
   } catch (const ks::Error& e) {
     int st = e.status;
@@ -1043,6 +1115,32 @@ ks_status ks_solve(ks_ctx* c, ks_mat* m, const ks_params* p, const double* b, do
   Out<double> xo(c, x, m->n, p->flags, (p->flags & KS_X0_GIVEN) != 0);
   solve_impl(c, m, p, false, bi.p, xo.p, hist, hist_cap, res, nullptr);
   xo.copy_back(c);
+  KS_HIPCK(hipStreamSynchronize(c->stream));
+  return KS_OK;
+  KS_API_CATCH(c)
+}
+
+ks_status ks_solve_noalloc(ks_ctx* c, ks_mat* m, const ks_params* p, const double* b, double* x,
+                           ks_result* res) {
+  if (!c || !m || !p || !b || !x) return KS_ERR_BAD_INPUT;
+  KS_API_TRY
+  set_device(c);
+  KS_CHECK(m->ctx == c, KS_ERR_BAD_INPUT, "matrix belongs to another context");
+  KS_CHECK(p->flags & KS_PTR_DEVICE, KS_ERR_BAD_INPUT,
+           "ks_solve_noalloc takes device pointers only (KS_PTR_DEVICE): host arrays would need staging buffers");
+  KS_CHECK(name_of(p->outer) == CG && name_of(p->inner) == NONE, KS_ERR_UNSUPPORTED,
+           "ks_solve_noalloc runs plain CG only (outer \"cg\", inner \"NO\"); use ks_solve");
+  KS_CHECK(m->n <= kNoAllocMaxRows, KS_ERR_BAD_INPUT,
+           "ks_solve_noalloc: n = " + std::to_string(m->n) + " exceeds its maximum of " +
+               std::to_string(kNoAllocMaxRows) + " rows (the size of its fixed scratch memory); use ks_solve");
+  KS_CHECK(c->scratch != nullptr, KS_ERR_UNSUPPORTED, "no scratch memory on device " + std::to_string(c->device));
+  static_assert(CgLevel<double>::scratch_doubles(kNoAllocMaxRows) <= 3 * kNoAllocMaxRows + 1024,
+                "scratch holds three padded vectors and the control state at the maximum size");
+  std::lock_guard<std::mutex> lk(scratch_table().call[c->device]);
+  CgLevel<double> L;
+  L.setup(c, m->view<double>(false), outer_cfg(p, 0), c->scratch);
+  // q is free once the solve has ended: it holds the final residual check
+  run_level(c, m, L, p, 0u, false, b, x, nullptr, 0, res, nullptr, L.q.p);
   KS_HIPCK(hipStreamSynchronize(c->stream));
   return KS_OK;
   KS_API_CATCH(c)

@@ -1514,6 +1514,14 @@ __global__ void k_gather(int64_t m, const int32_t* __restrict__ perm, const Tin*
     out[j] = (Tout)in[perm[j]];
 }
 
+// out[0..m) = v: clears memory with a kernel (no memset node), for state
+// that lives in memory the library does not allocate per call.
+template <class T>
+__global__ void k_fill(int64_t m, T* __restrict__ out, T v) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) out[j] = v;
+}
+
 template <class Tin, class Tout>
 __global__ void k_convert(int64_t m, const Tin* __restrict__ in, Tout* __restrict__ out, Gate g) {
   if (g.off()) return;

@@ -376,6 +376,9 @@ struct LevelCfg {
   int64_t hist_cap = 0;
 };
 
+// n rounded up to a multiple of 64 elements (256-byte aligned columns)
+constexpr int64_t pad64(int64_t n) { return (n + 63) & ~(int64_t)63; }
+
 struct Level {
   ks_ctx* c = nullptr;
   int64_t n = 0;
@@ -394,7 +397,20 @@ struct Level {
   Gate gdone() const { return Gate{outer_stop, &dctl.p->done}; }
   Gate gouter() const { return Gate{outer_stop, nullptr}; }
 
-  void setup_ctl(int R, int k) {
+  // Doubles of device memory setup_ctl(R, k, mem) takes from `mem`: the
+  // small arrays, then the control block on a 256-byte boundary.
+  static constexpr int64_t ctl_doubles(int R, int k) {
+    return pad64(small_doubles(R, k)) + pad64((int64_t)((sizeof(Ctl) + 7) / 8));
+  }
+  static constexpr int64_t small_doubles(int R, int k) {
+    const int64_t nH = R > 0 ? (int64_t)(R + 1) * R : 0;
+    const int64_t nsmall = R > 0 ? 6 * (int64_t)R : 0;
+    return nH + nsmall + 5 * (int64_t)(2 * k + 3) + 8;
+  }
+  // mem (optional): device memory of ctl_doubles(R, k) doubles that holds the
+  // small arrays and the control block instead of allocations (cleared here
+  // with a fill kernel); no history with it.
+  void setup_ctl(int R, int k, double* mem = nullptr) {
     hctl = Ctl{};
     hctl.status = 2;
     hctl.maxloop = cfg.maxloop;
@@ -405,10 +421,16 @@ struct Level {
     hctl.fix = cfg.fix;
     hctl.compat = cfg.compat;
     const int64_t nH = R > 0 ? (int64_t)(R + 1) * R : 0;
-    const int64_t nsmall = R > 0 ? 6 * (int64_t)R : 0;
-    const int64_t nk = 5 * (int64_t)(2 * k + 3);
-    arr.alloc((size_t)(nH + nsmall + nk + 8));
-    KS_HIPCK(hipMemsetAsync(arr.p, 0, arr.n * sizeof(double), c->stream));
+    const int64_t narr = small_doubles(R, k);
+    if (mem) {
+      KS_CHECK(cfg.hist_cap == 0, KS_ERR_BAD_INPUT, "no residual history without allocations");
+      arr.borrow(mem, (size_t)narr);
+      k_fill<double><<<grid_map(narr), kBlock, 0, c->stream>>>(narr, arr.p, 0.0);
+      KS_HIPCK(hipGetLastError());
+    } else {
+      arr.alloc((size_t)narr);
+      KS_HIPCK(hipMemsetAsync(arr.p, 0, arr.n * sizeof(double), c->stream));
+    }
     double* q = arr.p;
     if (R > 0) {
       hctl.H = q; q += nH;
@@ -429,7 +451,10 @@ struct Level {
       hctl.hist = hist.p;
       hctl.hist_cap = cfg.hist_cap;
     }
-    dctl.alloc(1);
+    if (mem)
+      dctl.borrow((Ctl*)(mem + pad64(narr)), 1);
+    else
+      dctl.alloc(1);
     upload_ctl();
   }
   void upload_ctl() { KS_HIPCK(hipMemcpyAsync(dctl.p, &hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream)); }
@@ -451,9 +476,22 @@ struct CgLevel : Level {
   DBuf<T> r, p, q, xown;
   T* x = nullptr;
   double* xout = nullptr;
-  DBuf<double> r64;  // outer with x0: residual source
-  void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf) {
+  // Doubles of device memory setup(..., mem) takes: r, p and q (each padded
+  // to 64 elements), then the small arrays and the control block.
+  static constexpr int64_t scratch_doubles(int64_t n) { return 3 * pad64(n) + ctl_doubles(0, 0); }
+  // mem (optional, FP64 only): device memory of scratch_doubles(n) doubles
+  // that holds every vector and the control state instead of allocations
+  void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf, double* mem = nullptr) {
     c = cx; A = a; n = a.n; cfg = cf;
+    if (mem) {
+      KS_CHECK(sizeof(T) == 8, KS_ERR_UNSUPPORTED, "borrowed CG state is FP64 only");
+      const int64_t ld = pad64(n);
+      r.borrow((T*)mem, n);
+      p.borrow((T*)(mem + ld), n);
+      q.borrow((T*)(mem + 2 * ld), n);
+      setup_ctl(0, 0, mem + 3 * ld);
+      return;
+    }
     r.alloc(n); p.alloc(n); q.alloc(n);
     if (sizeof(T) != 8) xown.alloc(n);
     setup_ctl(0, 0);
@@ -766,7 +804,6 @@ int launch_cgs_update(ks_ctx* c, int64_t n, const T* V, int64_t ldv, int nv, con
   return G;
 }
 
-inline int64_t pad64(int64_t n) { return (n + 63) & ~(int64_t)63; }
 
 // ---------------------------------------------------------- GMRES ------
 // gmres::solve (gmres.cpp:232-423); with `inner` set it is vpgmres::solve
