@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--spmv-config", default=None,
                     help="only the SpMV measurement of one configuration (its default layout, A and A^T), "
                          "one JSON line, no headline: e.g. zipf, for a profiler run")
+    ap.add_argument("--spmv-layouts", default="",
+                    help="with --spmv-config: also time these layouts (comma-separated names of "
+                         "solver_cpp_amd.api.Matrix.LAYOUTS, e.g. panels_merge,panels_wg,panels_wave)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB of one plain device copy at the end (counter calibration)")
     return ap.parse_args()
@@ -91,6 +94,13 @@ def main():
                           "fp32": spmv_gbps(A, n, nnz, 32), "fp64_transpose": spmv_gbps(A, n, nnz, 64, trans=True)}),
               flush=True)
         A.close()
+        for lay in filter(None, a.spmv_layouts.split(",")):
+            t = time.perf_counter()
+            A = api.Matrix(ctx, P["ptr"], P["col"], P["val"], fp32=True, layout=lay)
+            print(json.dumps({"suite": "spmv_layout", "config": a.spmv_config, "layout": lay,
+                              "setup_s": round(time.perf_counter() - t, 2), "spmv_kind": A.info()["spmv_kind"],
+                              "fp64": spmv_gbps(A, n, nnz, 64), "fp32": spmv_gbps(A, n, nnz, 32)}), flush=True)
+            A.close()
         calib_copy(a)
         ctx.close()
         return

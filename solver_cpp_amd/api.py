@@ -176,10 +176,13 @@ class Matrix:
     # matrices whose rows gather x from all over, merge-path for heavy-tailed
     # row lengths, CSR-stream on workgroup tiles otherwise), CSR-stream on
     # workgroup ("stream" / "wg") or wavefront tiles, lanes-per-row vector
-    # kernel, merge-path, column panels
+    # kernel, merge-path, column panels (each panel on the library's choice,
+    # or on a named layout: "panels_wg", "panels_wave", "panels_merge")
     LAYOUTS = {"auto": 0, "stream": _abi.KS_MAT_WG_TILES, "wg": _abi.KS_MAT_WG_TILES,
                "wave": _abi.KS_MAT_WAVE_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE,
-               "panels": _abi.KS_MAT_PANELS}
+               "panels": _abi.KS_MAT_PANELS, "panels_wg": _abi.KS_MAT_PANELS | _abi.KS_MAT_WG_TILES,
+               "panels_wave": _abi.KS_MAT_PANELS | _abi.KS_MAT_WAVE_TILES,
+               "panels_merge": _abi.KS_MAT_PANELS | _abi.KS_MAT_MERGE}
 
     def __init__(self, ctx, ptr, col, val, transpose=False, fp32=False, spd=False, layout="auto",
                  borrow=False):

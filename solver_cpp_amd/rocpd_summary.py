@@ -4,6 +4,7 @@
   python -m solver_cpp_amd.rocpd_summary pmc <results.db> [--kernel SUBSTR]
   python -m solver_cpp_amd.rocpd_summary traffic <fetch.db> --write-db <write.db>
          --kernel NAME --copy-bytes B
+  python -m solver_cpp_amd.rocpd_summary l2 <results.db> [--kernel SUBSTR]
 
 `traffic` turns one FETCH_SIZE run and one WRITE_SIZE run into HBM bytes per
 launch of one kernel: each counter is scaled by what it reads for the plain
@@ -17,7 +18,11 @@ launches that returned at once are left out) is reported, with a final machine-r
 `trace` lists kernels by total time (calls, total and mean duration in us;
 dispatches faster than --active-us are also reported separately: those are
 launches whose device-side stop flag made them return at once).  `pmc`
-lists the mean counter value per dispatch for each kernel.
+lists the median and largest counter value per dispatch for each kernel
+(FETCH_SIZE / WRITE_SIZE in KiB, other counters as counts).  `l2` takes a
+run with TCC_HIT_sum, TCC_MISS_sum and TCC_REQ_sum and lists, per kernel,
+the L2 hits, misses and requests per working dispatch and the hit rate
+hits / (hits + misses).
 """
 import argparse
 import collections
@@ -56,11 +61,35 @@ def pmc(db, match=None):
     agg = collections.defaultdict(list)
     for k, c, v in rows:
         agg[(short(k), c)].append(v)
-    out = [f"{'kernel':70s} {'counter':12s} {'dispatches':>10s} {'median_KiB':>12s} {'max_KiB':>12s}"]
+    out = [f"{'kernel':70s} {'counter':12s} {'dispatches':>10s} {'median':>12s} {'max':>12s}"]
     for (k, c), vs in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
         if match and match not in k:
             continue
         out.append(f"{k[:70]:70s} {c:12s} {len(vs):10d} {statistics.median(vs):12.1f} {max(vs):12.1f}")
+    return "\n".join(out)
+
+
+def l2(db, match=None, min_req=1000.0):
+    """Per kernel: TCC hits, misses and requests summed over its dispatches,
+    divided by its working dispatches (those with at least min_req requests:
+    a gated launch that returned at once adds next to nothing to the sums),
+    and the hit rate hits / (hits + misses)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    tot = collections.defaultdict(float)
+    work = collections.Counter()
+    for k, c, v in rows:
+        k = short(k)
+        tot[(k, c)] += v
+        if c == "TCC_REQ_sum" and v >= min_req:
+            work[k] += 1
+    out = [f"{'kernel':70s} {'dispatches':>10s} {'hit_M':>9s} {'miss_M':>9s} {'req_M':>9s} {'hit_rate':>8s}"]
+    for k, nd in sorted(work.items(), key=lambda kv: -tot[(kv[0], "TCC_REQ_sum")]):
+        if match and match not in k:
+            continue
+        h, m, q = (tot[(k, c)] / nd for c in ("TCC_HIT_sum", "TCC_MISS_sum", "TCC_REQ_sum"))
+        out.append(f"{k[:70]:70s} {nd:10d} {h / 1e6:9.2f} {m / 1e6:9.2f} {q / 1e6:9.2f} "
+                   f"{h / max(h + m, 1.0):8.3f}")
     return "\n".join(out)
 
 
@@ -92,7 +121,7 @@ def traffic(fetch_db, write_db, kernel, copy_bytes, stat="median"):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "pmc", "traffic"])
+    ap.add_argument("mode", choices=["trace", "pmc", "traffic", "l2"])
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=20)
     ap.add_argument("--active-us", type=float, default=10.0)
@@ -105,6 +134,8 @@ def main():
         print(trace(a.db, a.top, a.active_us, a.kernel))
     elif a.mode == "pmc":
         print(pmc(a.db, a.kernel))
+    elif a.mode == "l2":
+        print(l2(a.db, a.kernel))
     else:
         print(traffic(a.db, a.write_db, a.kernel, a.copy_bytes, a.stat))
 

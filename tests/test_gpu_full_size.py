@@ -15,6 +15,13 @@ that hold at any size (the oracle's own runs are kept to small sizes).
   unscaled true residuals of both solves, 64 central finite differences
   along relative value directions, and the oracle's FP64 adjoint solve of
   the same system on the host.
+- The adjoints of the 1024^2 anisotropic diffusion matrix (SPD: forward
+  VPCG with an FP32 inner CG, backward CG + SDDMM) and of the 192^3
+  convection-diffusion matrix at Peclet 0.8 (forward mixed VPGMRES(20),
+  backward GMRES(20) on the explicit transpose), to 1e-14: 64 central
+  finite differences each, eps = 1e-6 (symmetric edge perturbations of the
+  conductivities for the SPD case, which keep A SPD; seeded N(0,1) value
+  directions for the other), every direction within 1e-6 relative.
 - SpMV (cudaFunction.cu:127-156, blas.cpp:43-129) and the explicit A^T
   (cudaFunction.cu:896-931) at the full sizes of the 160^3 Poisson, 192^3
   convection-diffusion and 4,000,000-row Zipf matrices (the last on column
@@ -190,6 +197,56 @@ def test_zipf_adjoint_full_size(ctx, oracle):
     fdx.close()
     print(f"[zipf-adjoint] 64 FD directions in {time.perf_counter() - t4:.1f} s, max rel err {worst:.2e}",
           flush=True)
+    assert worst <= 1e-6
+    sysm.A.close()
+
+
+@pytest.mark.parametrize("cfg", ["aniso2d", "convdiff3d_pe08"])
+def test_adjoint_fd_full_size(ctx, cfg):
+    import torch
+    from solver_cpp_amd import autograd as AG
+    from solver_cpp_amd import gen
+    Pb = gen.problem(cfg)
+    ptr, col, val = Pb["ptr"], Pb["col"], Pb["val"]
+    nnz = len(col)
+    b = torch.from_numpy(Pb["b"]).cuda()
+    xt = torch.from_numpy(Pb["x_target"]).cuda()
+    if cfg == "aniso2d":
+        fwd = dict(outer_eps=1e-14, outer_maxloop=20000, inner_maxloop=10, inner_precision=32)
+        cg = dict(outer_eps=1e-14, outer_maxloop=100000)
+        sysm = AG.SparseSystem(ctx, ptr, col, spd=True, solver="vpcg", inner="cg", fp32=True, params=fwd,
+                               adjoint_solver="cg", adjoint_params=dict(cg))
+        fd_args = ("cg", "NO", cg)
+    else:
+        fwd = dict(outer_eps=1e-14, outer_maxloop=20000, outer_restart=21, inner_maxloop=11, inner_restart=12,
+                   inner_eps=0.0, inner_precision=32)
+        sysm = AG.SparseSystem(ctx, ptr, col, spd=False, solver="vpgmres", inner="gmres", fp32=True, params=fwd,
+                               adjoint_solver="gmres",
+                               adjoint_params=dict(outer_eps=1e-14, outer_maxloop=100000, outer_restart=21))
+        fd_args = ("vpgmres", "gmres", fwd)
+    vals = torch.from_numpy(val).cuda().requires_grad_(True)
+    bb = b.clone().requires_grad_(True)
+    x = sysm.solve(vals, bb)
+    (0.5 * ((x - xt) ** 2).sum()).backward()
+    assert sysm.last_forward["status"] == 0 and sysm.last_backward["status"] == 0
+    spd = cfg == "aniso2d"
+    fdx = AG.CentralFD(ctx, ptr, col, vals.detach(), b, xt, 1e-6, fd_args[0], fd_args[1], params=fd_args[2],
+                       spd=spd)
+    rng = np.random.default_rng(42 if spd else 54)
+    worst = 0.0
+    t0 = time.perf_counter()
+    for _ in range(64):
+        if spd:
+            dv = torch.from_numpy(gen.aniso2d_values(Pb["grid"], Pb["grid"], rng.standard_normal(len(Pb["kappa"]))))
+        else:
+            dv = torch.from_numpy(rng.standard_normal(nnz))
+        dv = dv.cuda()
+        fd = fdx(dv)
+        ad = float(vals.grad @ dv)
+        worst = max(worst, abs(fd - ad) / abs(ad))
+    fdx.close()
+    print(f"[adjoint-fd] {cfg}: forward {sysm.last_forward['iters']} it, adjoint {sysm.last_backward['iters']} it, "
+          f"64 FD directions in {time.perf_counter() - t0:.1f} s, max rel err {worst:.2e}", flush=True)
     assert worst <= 1e-6
     sysm.A.close()
 
