@@ -178,7 +178,7 @@ struct Plan {
   bool wg64 = true, wg32 = true;  // tile set of the FP64 / FP32 SpMV
   int L = 8;
   TileSet wave, wg, seg;
-  DBuf<double> carry;  // merge path: 2 * segments carries (FP64 or FP32)
+  DBuf<double> carry;  // merge path: carries and heads, 2 x 2 values per segment (FP64 or FP32)
 };
 
 template <class T>
@@ -392,22 +392,26 @@ int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const 
   return G;
 }
 
-// Merge-path SpMV: carries of the split rows, then the segments.
+// Merge-path SpMV: one pass over the segments, then the rows that span
+// segments.  Returns the number of reduction partials written (the main
+// kernel's workgroups, then the fix-up kernel's).
 template <class T, int NX, int NV, class Epi>
 int spmv_merge(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
                double bytes) {
   const int64_t nslot = (A.ntiles + kBlock / 64 - 1) / (kBlock / 64);
   c->prof.begin(c->stream, cls);
-  const int Gc = grid_rows(c, k_merge_carry<T, NX>, nslot);
-  k_merge_carry<T, NX><<<Gc, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
-                                                    A.carry, g);
-  KS_HIPCK(hipGetLastError());
-  const int G = grid_rows(c, k_spmv_merge<T, NX, NV, Epi>, nslot);
+  int G = grid_rows(c, k_spmv_merge<T, NX, NV, Epi>, nslot);
+  if (G > kMaxGrid - kMergeFixGrid) G = (kMaxGrid - kMergeFixGrid) & ~7;
   k_spmv_merge<T, NX, NV, Epi><<<G, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0,
                                                             x1, A.carry, epi, g, red(c), stat(c, cls, bytes));
+  KS_HIPCK(hipGetLastError());
+  int64_t gf = (A.ntiles + kBlock - 1) / kBlock;
+  const int Gf = (int)(gf < 1 ? 1 : gf > kMergeFixGrid ? kMergeFixGrid : gf);
+  k_merge_fix<T, NX, NV, Epi><<<Gf, kBlock, 0, c->stream>>>(A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.carry, epi, g,
+                                                            RedArgs{red(c).part + G});
   c->prof.end(c->stream);
   KS_HIPCK(hipGetLastError());
-  return G;
+  return G + Gf;
 }
 
 // One SpMV launch on A's own layout (no finisher); returns the grid size.
@@ -546,7 +550,7 @@ inline void make_plan(ks_ctx* c, Plan& pl, const int32_t* hp, int64_t n, int64_t
   if (merge) {
     std::vector<int32_t> t, z;
     merge_segments(hp, n, nnz, kMergeItems, t, z);
-    pl.carry.alloc(2 * (t.size() - 1));
+    pl.carry.alloc(4 * (t.size() - 1));
     upload_tiles(c, pl.seg, t, z);
     return;
   }

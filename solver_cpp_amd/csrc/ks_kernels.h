@@ -10,7 +10,7 @@
 //    per-workgroup partials in a fixed order and runs the scalar step);
 //  * CSR SpMV: CSR-stream tiles (k_spmv_stream per workgroup, k_spmv_wave
 //    per wavefront) that stage values and column indices in LDS and sum one
-//    row per lane; a merge-path variant (k_merge_carry + k_spmv_merge); and
+//    row per lane; a merge-path variant (k_spmv_merge + k_merge_fix); and
 //    a lanes-per-row kernel (k_spmv).  Work is assigned to XCDs in
 //    contiguous ranges so x gathers hit the XCD's own L2.  On workgroup
 //    tiles an epilogue can also take work that needs no global
@@ -750,88 +750,50 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(int64_t n, const int32_t* __re
 // The merge path of the row ends (ptr[1..n]) and the entries (0..nnz) is cut
 // into segments of kMergeItems items (Merrill & Garland, "Merge-based
 // parallel sparse matrix-vector multiplication", SC16); segment s starts at
-// path point (segr[s], segj[s]) (computed on the host).  A row belongs to the
-// segment in which it ends; the entries of a row that lie in earlier segments
-// are summed by those segments as carries: carry[s] is the partial sum of the
-// row still open at the end of segment s (k_merge_carry).  k_spmv_merge then
-// sums each segment's own rows from LDS products, a row continued from
-// earlier segments starting from its carries in segment order, so every row
-// total is formed in a fixed order and passes once through the epilogue.
+// path point (segr[s], segj[s]) (computed on the host), so it holds at most
+// kMergeItems entries, [segj[s], segj[s+1]).  A row belongs to the segment in
+// which it ends.  One pass over the entries (k_spmv_merge) leaves, per
+// segment, up to two partial sums: carry[s], the part of the row still open
+// at the segment's end, and head[s], the part of the row continued from
+// earlier segments that ends here.  Every other row of the segment starts
+// and ends in it and passes through the epilogue in the same pass.
+// k_merge_fix then forms each continued row's total as (its earlier
+// segments' carries, added in segment order) + head, and runs its
+// epilogue - so every row total forms in a fixed order.
 constexpr int kMergeItems = 512;
 constexpr int kMergeUnroll = kMergeItems / 64;
+// workgroups of k_merge_fix (its reduction partials follow the main
+// kernel's in the partial buffer)
+constexpr int kMergeFixGrid = 64;
 
+// Register budget of k_spmv_merge: two segments' entries live at once (the
+// current one's and the next one's loads).
 template <class T, int NX>
-__global__ __launch_bounds__(kBlock) void k_merge_carry(int64_t n, int64_t nseg, const int32_t* __restrict__ segr,
-                                                        const int32_t* __restrict__ segj,
-                                                        const int32_t* __restrict__ ptr,
-                                                        const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                        const T* __restrict__ x0, const T* __restrict__ x1,
-                                                        T* __restrict__ carry, Gate g) {
-  if (g.off()) return;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  RowBlocks rbs{nseg};
-  int64_t s, end, step;
-  rbs.range(s, end, step, w, kBlock / 64);
-  // a wavefront's segments follow one another: the next segment's bounds and
-  // row start are loaded while the current one is summed, so each segment
-  // waits for its entries and gathers only
-  int32_t r = 0, js = 0, je = 0, pr = 0;
-  if (s < end) {
-    r = segr[s + 1]; js = segj[s]; je = segj[s + 1];
-    if (r < n) pr = ptr[r];
-  }
-  for (; s < end; s += step) {
-    const int64_t sn = s + step;
-    int32_t nr = 0, njs = 0, nje = 0, npr = 0;
-    if (sn < end) { nr = segr[sn + 1]; njs = segj[sn]; nje = segj[sn + 1]; }
-    T s0 = 0, s1 = 0;
-    if (r < n) {
-      const int32_t a = pr > js ? pr : js;
-      int32_t k = a + lane;
-      // two entries per lane in flight; each lane still adds its entries in
-      // increasing k, as one at a time would
-      for (; k + 64 < je; k += 128) {
-        const T v0 = ld_stream(val + k), v1 = ld_stream(val + k + 64);
-        const int c0 = ld_stream(col + k), c1 = ld_stream(col + k + 64);
-        const T g0 = x0[c0], g1 = x0[c1];
-        T h0 = 0, h1 = 0;
-        if (NX == 2) { h0 = x1[c0]; h1 = x1[c1]; }
-        s0 += v0 * g0;
-        if (NX == 2) s1 += v0 * h0;
-        s0 += v1 * g1;
-        if (NX == 2) s1 += v1 * h1;
-      }
-      if (k < je) {
-        const T v = ld_stream(val + k);
-        const int c = ld_stream(col + k);
-        s0 += v * x0[c];
-        if (NX == 2) s1 += v * x1[c];
-      }
-    }
-    if (sn < end && nr < n) npr = ptr[nr];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      s0 += __shfl_xor(s0, off, 64);
-      if (NX == 2) s1 += __shfl_xor(s1, off, 64);
-    }
-    if (lane == 0) {
-      carry[s] = s0;
-      if (NX == 2) carry[nseg + s] = s1;
-    }
-    r = nr; js = njs; je = nje; pr = npr;
-  }
-}
+constexpr int kMergeWaves = sizeof(T) == 4 ? 6 : (NX == 2 ? 4 : 5);
 
+// One pass of the merge path.  Each wavefront takes its segments in order
+// and keeps the next segment's bounds and entries in flight while it
+// gathers, stages and sums the current one:
+//   1. gathers for the current entries; loads of the next entries, the
+//      bounds two segments ahead and the current rows' pointers;
+//   2. products v * x (rounded, no FMA) of all the segment's entries to LDS;
+//   3. carry[s]: lane l adds the open row's products l, l+64, ... in order,
+//      then a fixed shuffle tree;
+//   4. the segment's rows from LDS, tpr lanes per row (a fixed tree); the
+//      continued row's sum goes to head[s] instead of the epilogue.
+// carry and head hold NX values per segment: carry[v * nseg + s],
+// head[v * nseg + s] = carry[(NX + v) * nseg + s].
 template <class T, int NX, int NV, class Epi>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamWaves<T, NX, NV>))) void k_spmv_merge(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMergeWaves<T, NX>))) void k_spmv_merge(
     int64_t n, int64_t nseg, const int32_t* __restrict__ segr, const int32_t* __restrict__ segj,
     const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const T* __restrict__ val,
-    const T* __restrict__ x0, const T* __restrict__ x1, const T* __restrict__ carry, Epi epi, Gate g, RedArgs ra,
+    const T* __restrict__ x0, const T* __restrict__ x1, T* __restrict__ carry, Epi epi, Gate g, RedArgs ra,
     StatArgs st) {
   if (g.off()) return;
   st.count();
   epi.prep();
   constexpr int kW = kBlock / 64;
+  constexpr int U = kMergeUnroll;
   __shared__ T prod[NX][kW][kMergeItems];
   double acc[NV > 0 ? NV : 1];
 #pragma unroll
@@ -839,49 +801,100 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   T* p0 = prod[0][w];
   T* p1 = prod[NX - 1][w];
+  T* head = carry + (int64_t)NX * nseg;
   RowBlocks rbs{nseg};
   int64_t s, end, step;
   rbs.range(s, end, step, w, kW);
-  for (; s < end; s += step) {
-    const int32_t r0 = segr[s], r1 = segr[s + 1], js = segj[s], je = segj[s + 1];
-    // own entries: up to the start of the open row r1 (summed as carry[s])
-    int32_t e = je;
-    if (r1 < n) {
-      const int32_t pr1 = ptr[r1];
-      e = pr1 > js ? pr1 : js;
-    }
-    const int cnt = e - js;
+  // current segment: bounds and entries; next segment: bounds and entries
+  // in flight; the one after: bounds
+  int32_t r0 = 0, r1 = 0, js = 0, je = 0, nr0 = 0, nr1 = 0, njs = 0, nje = 0;
+  T v[U], vn[U];
+  int32_t c[U], cn[U];
 #pragma unroll
-    for (int q = 0; q < kMergeUnroll; ++q) {
-      const int k = lane + q * 64;
+  for (int q = 0; q < U; ++q) { v[q] = vn[q] = T(0); c[q] = cn[q] = 0; }
+  if (s < end) {
+    r0 = segr[s]; r1 = segr[s + 1]; js = segj[s]; je = segj[s + 1];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int32_t k = js + lane + 64 * q;
+      if (k < je) { v[q] = ld_stream(val + k); c[q] = ld_stream(col + k); }
+    }
+  }
+  if (s + step < end) { nr0 = segr[s + step]; nr1 = segr[s + step + 1]; njs = segj[s + step]; nje = segj[s + step + 1]; }
+  for (; s < end; s += step) {
+    const bool more = s + step < end, more2 = s + 2 * step < end;
+    const int cnt = je - js;
+    // 1. gathers; then the next segment's entries and the bounds after it
+    T gx0[U], gx1[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      gx0[q] = gx1[q] = T(0);
+      if (lane + 64 * q < cnt) {
+        gx0[q] = x0[c[q]];
+        if (NX == 2) gx1[q] = x1[c[q]];
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const int32_t k = njs + lane + 64 * q;
+        if (k < nje) { vn[q] = ld_stream(val + k); cn[q] = ld_stream(col + k); }
+      }
+    }
+    int32_t mr0 = 0, mr1 = 0, mjs = 0, mje = 0;
+    if (more2) {
+      mr0 = segr[s + 2 * step]; mr1 = segr[s + 2 * step + 1]; mjs = segj[s + 2 * step]; mje = segj[s + 2 * step + 1];
+    }
+    // the row pointers of the segment's first 64 rows (row r0 + lane), and
+    // the open row's start
+    const int nr = r1 - r0;
+    const int32_t prow = ptr[r0 + (lane < nr ? lane : nr)];
+    const int32_t pr1 = r1 < n ? ptr[r1] : je;
+    // 2. products to LDS
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int k = lane + 64 * q;
       if (k < cnt) {
-        const T v = ld_stream(val + js + k);
-        const int c = ld_stream(col + js + k);
-        p0[k] = v * x0[c];
-        if (NX == 2) p1[k] = v * x1[c];
+        p0[k] = v[q] * gx0[q];
+        if (NX == 2) p1[k] = v[q] * gx1[q];
       }
     }
     wave_sync();
-    const int nr = r1 - r0;
+    // 3. the open row's part: carry[s]
+    {
+      T a0 = 0, a1 = 0;
+      if (r1 < n) {
+        const int a = (pr1 > js ? pr1 : js) - js;
+        for (int k = a + lane; k < cnt; k += 64) {
+          a0 += p0[k];
+          if (NX == 2) a1 += p1[k];
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        if (NX == 2) a1 += __shfl_xor(a1, off, 64);
+      }
+      if (lane == 0) {
+        carry[s] = a0;
+        if (NX == 2) carry[nseg + s] = a1;
+      }
+    }
+    // 4. the segment's rows r0 .. r1 - 1
+    const int ecnt = (pr1 < je ? pr1 : je) - js;  // entries of rows that end here
     int tpr = 1;
-    while (tpr < 64 && tpr * 2 * nr <= 64 && tpr * 8 * nr < cnt) tpr *= 2;
+    while (tpr < 64 && tpr * 2 * nr <= 64 && tpr * 8 * nr < ecnt) tpr *= 2;
     const int sub = lane % tpr;
     for (int base = 0; base < nr; base += 64 / tpr) {
       const int grp = base + lane / tpr;
       const bool live = grp < nr;
-      const int32_t r = r0 + grp;
-      T a0 = 0, a1 = 0, c0 = 0, c1 = 0;
+      int32_t pa = __shfl(prow, grp < 63 ? grp : 63, 64), pb = __shfl(prow, grp < 62 ? grp + 1 : 63, 64);
+      if (live && grp >= 62) {
+        pa = ptr[r0 + grp];
+        pb = ptr[r0 + grp + 1];
+      }
+      T a0 = 0, a1 = 0;
       if (live) {
-        const int32_t pa = ptr[r], pb = ptr[r + 1];
-        if (grp == 0 && pa < js && sub == 0) {
-          // carries of the earlier segments that ended inside row r0, in order
-          int64_t f = s;
-          while (f > 0 && segr[f] == r0) --f;
-          for (int64_t q = f; q < s; ++q) {
-            c0 += carry[q];
-            if (NX == 2) c1 += carry[nseg + q];
-          }
-        }
         const int sa = (pa > js ? pa : js) - js, sb = pb - js;
         for (int k = sa + sub; k < sb; k += tpr) {
           a0 += p0[k];
@@ -892,9 +905,52 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStreamW
         a0 += __shfl_xor(a0, off, 64);
         if (NX == 2) a1 += __shfl_xor(a1, off, 64);
       }
-      if (live && sub == 0) epi.row(r, c0 + a0, c1 + a1, acc);
+      if (live && sub == 0) {
+        if (grp == 0 && pa < js) {  // continued from earlier segments: k_merge_fix finishes it
+          head[s] = a0;
+          if (NX == 2) head[nseg + s] = a1;
+        } else {
+          epi.row(r0 + grp, a0, a1, acc);
+        }
+      }
     }
     wave_sync();
+    r0 = nr0; r1 = nr1; js = njs; je = nje;
+    nr0 = mr0; nr1 = mr1; njs = mjs; nje = mje;
+#pragma unroll
+    for (int q = 0; q < U; ++q) { v[q] = vn[q]; c[q] = cn[q]; }
+  }
+  if constexpr (NV > 0) block_store<NV>(acc, ra);
+}
+
+// The rows continued from earlier segments that end in segment s: total =
+// (carries of the segments f .. s-1 whose open row it was, in order) +
+// head[s], then the epilogue; the reduction partials go to this kernel's own
+// slots (ra points past the main kernel's).
+template <class T, int NX, int NV, class Epi>
+__global__ __launch_bounds__(kBlock) void k_merge_fix(int64_t n, int64_t nseg, const int32_t* __restrict__ segr,
+                                                      const int32_t* __restrict__ segj,
+                                                      const int32_t* __restrict__ ptr,
+                                                      const T* __restrict__ carry, Epi epi, Gate g, RedArgs ra) {
+  if (g.off()) return;
+  epi.prep();
+  double acc[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < (NV > 0 ? NV : 1); ++i) acc[i] = 0.0;
+  const T* head = carry + (int64_t)NX * nseg;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += stride) {
+    const int32_t r0 = segr[s], r1 = segr[s + 1];
+    if (r0 < r1 && ptr[r0] < segj[s]) {
+      int64_t f = s;
+      while (f > 0 && segr[f] == r0) --f;
+      T c0 = 0, c1 = 0;
+      for (int64_t q = f; q < s; ++q) {
+        c0 += carry[q];
+        if (NX == 2) c1 += carry[nseg + q];
+      }
+      epi.row(r0, c0 + head[s], NX == 2 ? c1 + head[nseg + s] : T(0), acc);
+    }
   }
   if constexpr (NV > 0) block_store<NV>(acc, ra);
 }
