@@ -69,7 +69,7 @@ typedef enum ks_status {
  * 2045 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 509) */
 #define KS_MAT_WG_TILES 128u   /* workgroup tiles (the default) */
 #define KS_MAT_WAVE_TILES 512u /* wavefront tiles */
-#define KS_MAT_PANELS   1024u /* column panels: the SpMV runs as 8 passes over
+#define KS_MAT_PANELS   1024u /* column panels: the SpMV runs as 10 passes over
                                  column slices of x, each small enough for an
                                  XCD's L2 (chosen automatically, with no layout
                                  flag, for n >= 2^21 rows when over half the

@@ -129,9 +129,12 @@ bool merge_choice(uint32_t flags, int64_t n, int64_t nnz, int64_t max_row) {
 // columns from their row, as with random columns or a scrambled ordering (a
 // banded matrix, or one whose rows add a few far columns to a local band,
 // gathers mostly near its rows and keeps one pass) - the SpMV runs as kPanels
-// passes over column slices of x (an x slice of n / 8 entries is 4 MB at
-// n = 4M in FP64, an XCD's L2).
-constexpr int kPanels = 8;
+// passes over column slices of x small enough for an XCD's 4 MB L2 (n / 10
+// entries: 3.2 MB at n = 4M in FP64).  Ten is the fastest count measured on
+// the 4M-row Zipf matrix (FP64 1.89 ms against 1.96 / 1.93 / 1.91 / 2.07 ms
+// with 6 / 8 / 12 / 16 panels): fewer passes miss L2 more on the gathers,
+// more passes re-read every row's pointer and partial sum more often.
+constexpr int kPanels = 10;
 bool panel_choice(uint32_t flags, int64_t n, const int32_t* hp, const int32_t* hc) {
   if (flags & KS_MAT_PANELS) return true;
   if (flags & (KS_MAT_VECTOR | KS_MAT_WG_TILES | KS_MAT_WAVE_TILES | KS_MAT_MERGE)) return false;
