@@ -548,37 +548,47 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
     // dots (y, V_v) over this workgroup's own rows, which it has just
     // written (visible to the whole workgroup after the loop's last
     // barrier; still in this XCD's L2), so y is not read back from HBM and
-    // no separate projection launch runs.  Each wavefront takes every
-    // fourth of the workgroup's tiles and reads 16-byte chunks of rows
-    // (y, V and ldv 16-byte aligned, checked by the host); a tile's rows
-    // outside its aligned chunks go one per lane.  Eight columns per pass,
-    // each pass's partials to part[v * kMaxGrid + workgroup].
+    // no separate projection launch runs.  The columns are split among the
+    // wavefronts (wavefront w takes columns w, w + 4, ..., four at a time)
+    // and each walks all of the workgroup's tiles, the next tile's bounds
+    // loaded while the current one is summed; rows are read in 16-byte
+    // chunks (y, V and ldv 16-byte aligned, checked by the host), a tile's
+    // rows outside its aligned chunks one per lane.  Each wavefront sums its
+    // columns over its lanes and writes them to part[v * kMaxGrid +
+    // workgroup] itself, so no barrier follows.
     static_assert(NV == 0 && NX == 1, "projection epilogue takes a plain SpMV");
     using VT = typename Chunk<T, 16 / sizeof(T)>::type;
     constexpr int W = 16 / sizeof(T);
     constexpr int kW = kBlock / 64;
+    constexpr int CP = 4;  // columns per wavefront per pass
     const int lane = tid & 63, wv = tid >> 6;
     int64_t u0, uend, ustep;
     rbs.range(u0, uend, ustep);
     const int nv = epi.nv;
     const int64_t ldv = epi.ldv;
-    for (int v0 = 0; v0 < nv; v0 += 8) {
-      double pa[8];
+    for (int c0 = wv; c0 < nv; c0 += kW * CP) {
+      double pa[CP];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) pa[j] = 0.0;
-      const T* vb = epi.V + (int64_t)v0 * ldv;
-      for (int64_t u = u0 + (int64_t)wv * ustep; u < uend; u += (int64_t)kW * ustep) {
-        const int64_t q0 = tiles[u], q1 = tiles[u + 1];
+      for (int j = 0; j < CP; ++j) pa[j] = 0.0;
+      const T* vb = epi.V + (int64_t)c0 * ldv;
+      int64_t q0 = 0, q1 = 0;
+      if (u0 < uend) { q0 = tiles[u0]; q1 = tiles[u0 + 1]; }
+      for (int64_t u = u0; u < uend; u += ustep) {
+        int64_t n0 = 0, n1 = 0;
+        if (u + ustep < uend) { n0 = tiles[u + ustep]; n1 = tiles[u + ustep + 1]; }
         const int64_t a = (q0 + W - 1) / W * W, b = q1 / W * W;
         for (int64_t r = a + (int64_t)W * lane; r + W <= b; r += 64 * W) {
           const VT yv = *reinterpret_cast<const VT*>(epi.y + r);
+          VT xv[CP];
 #pragma unroll
-          for (int j = 0; j < 8; ++j)
-            if (v0 + j < nv) {
-              const VT xv = *reinterpret_cast<const VT*>(vb + (int64_t)j * ldv + r);
+          for (int j = 0; j < CP; ++j)
+            if (c0 + kW * j < nv) xv[j] = *reinterpret_cast<const VT*>(vb + (int64_t)(kW * j) * ldv + r);
+#pragma unroll
+          for (int j = 0; j < CP; ++j)
+            if (c0 + kW * j < nv) {
               double t = 0.0;
 #pragma unroll
-              for (int e = 0; e < W; ++e) t += (double)yv[e] * (double)xv[e];
+              for (int e = 0; e < W; ++e) t += (double)yv[e] * (double)xv[j][e];
               pa[j] += t;
             }
         }
@@ -591,12 +601,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
         if (row >= 0) {
           const double wr = (double)epi.y[row];
 #pragma unroll
-          for (int j = 0; j < 8; ++j)
-            if (v0 + j < nv) pa[j] += wr * (double)vb[(int64_t)j * ldv + row];
+          for (int j = 0; j < CP; ++j)
+            if (c0 + kW * j < nv) pa[j] += wr * (double)vb[(int64_t)(kW * j) * ldv + row];
         }
+        q0 = n0; q1 = n1;
       }
-      if (v0 > 0) __syncthreads();  // block_store's staging is reused
-      block_store<8>(pa, RedArgs{ra.part + (size_t)v0 * kMaxGrid});
+#pragma unroll
+      for (int j = 0; j < CP; ++j) {
+        const double v = wave_sum(pa[j]);
+        if (lane == 0 && c0 + kW * j < nv) ra.part[(size_t)(c0 + kW * j) * kMaxGrid + blockIdx.x] = v;
+      }
     }
   }
 }
