@@ -25,9 +25,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # dominant kernel of the headline solve (the inner FP32 GMRES step's SpMV
-# with the Gram-Schmidt projection fused behind it), and the committed counter
+# with the Gram-Schmidt projection fused behind it, reading int16 column
+# offsets: the stencil's columns lie within 25,600 of their rows), and the committed counter
 # summary (two rocprofv3 --pmc runs, copy-calibrated) that gives its HBM traffic
-DOM_KERNEL = "ks::k_spmv_stream<float, 1, 0, ks::EpiDivProj<float> >"
+DOM_KERNEL = "ks::k_spmv_stream<float, 1, 0, ks::EpiDivProj<float>, short>"
 TRAFFIC_FILE = os.path.join("profiles", "k_spmv_stream_fp32_poisson160_traffic.txt")
 
 
@@ -138,9 +139,10 @@ def main():
     sec = (time.perf_counter() - t0) / a.steps
     # roofline of the dominant kernel from a separate pass of profiled solves:
     # every launch of it carries a kernel-timestamp event pair; its mean
-    # algorithmic bytes per launch as the library counts them: SpMV
-    # 8*nnz + 12*n + 4, plus 4*n*nv for the nv basis vectors the fused
-    # projection reads (nv = k + 1 at inner step k)
+    # algorithmic bytes per launch as the library counts them, for the index
+    # format the kernel reads: SpMV (4 + ib)*nnz + 12*n + 4 with ib bytes per
+    # column index (2: int16 offsets, 4: int32 columns), plus 4*n*nv for the
+    # nv basis vectors the fused projection reads (nv = k + 1 at inner step k)
     prof = [run("vpgmres", "gmres", profile=1 << dom, **mixed)[1] for _ in range(a.prof_steps)]
     cls = KERNEL_CLASSES[dom]
     ms_tot = sum(rr["prof"].get(cls, {}).get("ms", 0.0) for rr in prof)
@@ -149,6 +151,9 @@ def main():
     bytes_per = by_tot / max(launches, 1)
     ms_per = ms_tot / max(launches, 1)
     achieved = bytes_per / (ms_per * 1e-3) / 1e9 if launches else None
+    ib = A.index_bytes(False, 32)
+    # the same launches priced with int32 column indices (4 bytes per entry)
+    bytes_per32 = bytes_per + (4 - ib) * nnz
 
     out = {
         "metric": METRIC, "value": round(sec, 6), "unit": "s", "n_gpus": 1, "steps": a.steps,
@@ -164,6 +169,10 @@ def main():
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                        "traffic": tr, "traffic_source": TRAFFIC_FILE if tr else None,
                        "bytes_per_launch": bytes_per, "ms_per_launch": ms_per, "launches": launches,
+                       "index_bytes": ib,
+                       "csr_int32_equivalent": {"bytes_per_launch": bytes_per32,
+                                                "GBps": round(bytes_per32 / (ms_per * 1e-3) / 1e9, 1)
+                                                if launches else None},
                        "profiled_solves": a.prof_steps,
                        "profiled_s_per_solve": round(sum(rr["seconds"] for rr in prof) / max(len(prof), 1), 6)}
     out["solve_bytes_GBps"] = round(r0["bytes"] / r0["seconds"] / 1e9, 1)
@@ -359,9 +368,14 @@ def spmv_gbps(A, n, nnz, prec, trans=False, reps=30):
     A.spmv_bench(xx, yy, reps=3, trans=trans, precision=prec)
     ms = A.spmv_bench(xx, yy, reps=reps, trans=trans, precision=prec)
     sv = prec // 8
-    by = (sv + 4) * nnz + 4 * (n + 1) + 2 * sv * n
+    ib = A.index_bytes(trans, prec)
+    # bytes of the format the kernel reads (ib bytes per column index), and
+    # the int32-CSR bytes of the same SpMV
+    by = (sv + ib) * nnz + 4 * (n + 1) + 2 * sv * n
+    by32 = by + (4 - ib) * nnz
     return {"us": round(ms * 1e3, 2), "GBps": round(by / (ms * 1e-3) / 1e9, 1),
-            "frac_of_8TBps": round(by / (ms * 1e-3) / 8e12, 4)}
+            "frac_of_8TBps": round(by / (ms * 1e-3) / 8e12, 4), "index_bytes": ib,
+            "csr_int32_GBps": round(by32 / (ms * 1e-3) / 1e9, 1)}
 
 
 def suite(ctx, a):

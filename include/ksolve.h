@@ -75,6 +75,12 @@ typedef enum ks_status {
                                  flag, for n >= 2^21 rows when over half the
                                  entries lie more than n/8 columns from their
                                  row) */
+#define KS_MAT_COL32    2048u /* keep int32 column indices in the SpMV.  Without
+                                 it, a matrix whose entries all lie within
+                                 32767 columns of their row (stencils, banded
+                                 matrices) also gets int16 offsets col - row,
+                                 which the workgroup-tile SpMV reads instead
+                                 (2 bytes per entry less; same results) */
 #define KS_MAT_BORROW   256u /* with KS_PTR_DEVICE: use the caller's ptr/col/val
                                 in place (no copy); they must outlive the
                                 matrix, col and val must be 16-byte aligned,
@@ -176,6 +182,12 @@ void ks_mat_destroy(ks_mat* m);
  * two >= the mean row length, in [2, 64]), reported for every kind. */
 ks_status ks_mat_info(const ks_mat* m, int64_t* n, int64_t* nnz,
                       int32_t* spmv_kind, int32_t* lanes_per_row);
+/* Bytes per column index that the SpMV of A (trans = 0) or A^T (trans = 1)
+ * reads, in a precision (KS_F64 / KS_F32): 2 with int16 offsets (see
+ * KS_MAT_COL32), else 4.  No reference counterpart (the reference always
+ * reads int32 columns, cudaFunction.cu:127-156); the bench uses it for the
+ * SpMV's byte count. */
+ks_status ks_mat_index_bytes(const ks_mat* m, int32_t trans, int32_t precision, int32_t* bytes);
 
 /* ---- operators (the BLAS layer; tests and benchmarks call these) -------- */
 /* y = A x or y = A^T x.  Replaces cuda::MtxVec_mult (cudaFunction.cu:613-638,

@@ -29,6 +29,7 @@ KS_MAT_WG_TILES = 128
 KS_MAT_BORROW = 256
 KS_MAT_WAVE_TILES = 512
 KS_MAT_PANELS = 1024
+KS_MAT_COL32 = 2048
 
 KS_F64 = 64
 KS_F32 = 32

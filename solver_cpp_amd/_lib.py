@@ -33,6 +33,7 @@ _SIGS = {
     "ks_mat_get_transpose": (_i32, [_vp, _vp, _vp, _vp, _u32]),
     "ks_mat_destroy": (None, [_vp]),
     "ks_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i32), C.POINTER(_i32)]),
+    "ks_mat_index_bytes": (_i32, [_vp, _i32, _i32, C.POINTER(_i32)]),
     "ks_spmv": (_i32, [_vp, _i32, _i32, _vp, _vp, _u32]),
     "ks_spmv_bench": (_i32, [_vp, _i32, _i32, _vp, _vp, _i32, C.POINTER(_dbl)]),
     "ks_dot": (_i32, [_vp, _i64, _i32, _vp, _vp, C.POINTER(_dbl), _u32]),

@@ -177,8 +177,12 @@ class Matrix:
     # row lengths, CSR-stream on workgroup tiles otherwise), CSR-stream on
     # workgroup ("stream" / "wg") or wavefront tiles, lanes-per-row vector
     # kernel, merge-path, column panels (each panel on the library's choice,
-    # or on a named layout: "panels_wg", "panels_wave", "panels_merge")
-    LAYOUTS = {"auto": 0, "stream": _abi.KS_MAT_WG_TILES, "wg": _abi.KS_MAT_WG_TILES,
+    # or on a named layout: "panels_wg", "panels_wave", "panels_merge");
+    # "col32" / "wg_col32": the library's choice / workgroup tiles with int32
+    # column indices kept in the SpMV (without it, banded matrices are read
+    # with int16 offsets)
+    LAYOUTS = {"auto": 0, "col32": _abi.KS_MAT_COL32, "stream": _abi.KS_MAT_WG_TILES, "wg": _abi.KS_MAT_WG_TILES,
+               "wg_col32": _abi.KS_MAT_WG_TILES | _abi.KS_MAT_COL32,
                "wave": _abi.KS_MAT_WAVE_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE,
                "panels": _abi.KS_MAT_PANELS, "panels_wg": _abi.KS_MAT_PANELS | _abi.KS_MAT_WG_TILES,
                "panels_wave": _abi.KS_MAT_PANELS | _abi.KS_MAT_WAVE_TILES,
@@ -216,6 +220,13 @@ class Matrix:
         self.ctx.check(self.ctx.lib.ks_mat_info(self.h, C.byref(n), C.byref(nnz), C.byref(kind), C.byref(lanes)),
                        "info")
         return dict(n=n.value, nnz=nnz.value, spmv_kind=kind.value, lanes_per_row=lanes.value)
+
+    def index_bytes(self, trans=False, precision=64):
+        """Bytes per column index that this matrix's SpMV reads (2: int16
+        offsets col - row, 4: int32 columns)."""
+        b = C.c_int32()
+        self.ctx.check(self.ctx.lib.ks_mat_index_bytes(self.h, int(trans), precision, C.byref(b)), "index_bytes")
+        return b.value
 
     def update_values(self, val):
         av = _Arg(val, np.float64)

@@ -186,6 +186,11 @@ struct MatView {
   int64_t n = 0, nnz = 0;
   const int32_t* ptr = nullptr;
   const int32_t* col = nullptr;
+  // int16 offsets col - row, when the matrix has them (banded matrices) and
+  // this view's SpMV reads them (workgroup CSR-stream tiles); ib = bytes per
+  // column index that the SpMV reads
+  const int16_t* col16 = nullptr;
+  int ib = 4;
   const T* val = nullptr;
   int kind = 0;
   bool wg_tiles = false;
@@ -239,6 +244,7 @@ struct ks_mat {
   int64_t n = 0, nnz = 0;
   uint32_t flags = 0;
   ks::DBuf<int32_t> ptr, col;
+  ks::DBuf<int16_t> col16, tcol16;  // col - row of A and A^T (banded matrices)
   ks::DBuf<double> val;
   ks::DBuf<float> val32;
   bool has_t = false;
@@ -251,6 +257,12 @@ struct ks_mat {
 
   template <class T>
   ks::MatView<T> view(bool trans) const;
+  // hands the int16 offsets to a view whose SpMV reads them
+  template <class T>
+  void set_col16(ks::MatView<T>& v, bool trans) const {
+    const int16_t* p = trans ? tcol16.p : col16.p;
+    if (p && v.kind == 0 && v.wg_tiles) { v.col16 = p; v.ib = 2; }
+  }
 };
 
 template <>
@@ -263,6 +275,7 @@ inline ks::MatView<double> ks_mat::view<double>(bool trans) const {
   v.set_plan(pl);
   const ks::PanelSet& ps = trans ? tpanels : panels;
   if (ps.np) { v.np = ps.np; v.panel = ps.v64.data(); v.pacc = ps.acc64.p; v.kind = 3; }
+  set_col16(v, trans);
   return v;
 }
 template <>
@@ -275,6 +288,7 @@ inline ks::MatView<float> ks_mat::view<float>(bool trans) const {
   v.set_plan(pl);
   const ks::PanelSet& ps = trans ? tpanels : panels;
   if (ps.np && ps.val32.p) { v.np = ps.np; v.panel = ps.v32.data(); v.pacc = ps.acc32.p; v.kind = 3; }
+  set_col16(v, trans);
   return v;
 }
 
@@ -361,10 +375,11 @@ void launch_scalar(ks_ctx* c, const Op& op, Gate g) {
   KS_HIPCK(hipGetLastError());
 }
 
-// SpMV bytes: values + column indices + row pointers + x read once + y written.
+// SpMV bytes: values + column indices (ib bytes each) + row pointers + x read
+// once + y written.
 template <class T>
-inline double spmv_bytes(int64_t n, int64_t nnz) {
-  return (double)(sizeof(T) + 4) * nnz + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
+inline double spmv_bytes(int64_t n, int64_t nnz, int ib = 4) {
+  return (double)(sizeof(T) + ib) * nnz + 4.0 * (n + 1) + 2.0 * sizeof(T) * n;
 }
 
 template <class T, int L, int NV, class Epi>
@@ -380,7 +395,11 @@ template <class T, int NX, int NV, class Epi>
 int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
                 double bytes) {
   int G;
-  if (A.wg_tiles) {
+  if (A.wg_tiles && A.col16) {
+    G = grid_rows(c, k_spmv_stream<T, NX, NV, Epi, int16_t>, A.ntiles);
+    klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi, int16_t>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col16,
+            A.val, x0, x1, epi, g, red(c), stat(c, cls, bytes));
+  } else if (A.wg_tiles) {
     G = grid_rows(c, k_spmv_stream<T, NX, NV, Epi>, A.ntiles);
     klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
             epi, g, red(c), stat(c, cls, bytes));
@@ -452,7 +471,7 @@ struct EpiPanelLast {
 template <class T, int NV, class Epi>
 void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, double extra_bytes = 0) {
   const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
-  const double bytes = spmv_bytes<T>(A.n, A.nnz) + extra_bytes;
+  const double bytes = spmv_bytes<T>(A.n, A.nnz, A.ib) + extra_bytes;
   int G;
   if (A.np > 0) {
     // the bytes of the whole SpMV are counted with the last pass
@@ -479,7 +498,7 @@ void launch_spmv2(ks_ctx* c, const MatView<T>& A, const T* x1, const T* x2, T* y
     launch_spmv<T, 0>(c, A, x2, EpiStore<T>{y2}, g);
     return;
   }
-  const double bytes = spmv_bytes<T>(A.n, A.nnz) + 2.0 * sizeof(T) * A.n;
+  const double bytes = spmv_bytes<T>(A.n, A.nnz, A.ib) + 2.0 * sizeof(T) * A.n;
   if (A.kind == 2) spmv_merge<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);
   else spmv_stream<T, 2, 0>(c, A, x1, x2, EpiStore2<T>{y1, y2}, g, KS_K_BASIS, bytes);
 }

@@ -1,6 +1,7 @@
 // ks_device.hip - libksolve entry points that touch the GPU: context, matrix
 // (upload, FP32 copy, device transpose), BLAS-layer operators, ks_solve and
 // ks_adjoint.  All work is ordered on the context's HIP stream.
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <mutex>
@@ -191,6 +192,35 @@ void host_rows(int64_t n, F fn) {
   std::vector<std::thread> th;
   for (int64_t t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
   for (auto& x : th) x.join();
+}
+
+// 16-bit column offsets: the workgroup CSR-stream SpMV reads col - row as an
+// int16 instead of the int32 column (2 bytes per entry less; stencil
+// matrices), unless KS_MAT_COL32.  Every offset must lie in [-32767, 32767]:
+// A^T's offsets are A's negated, so both fit.
+bool offsets_fit16(int64_t n, const int32_t* hp, const int32_t* hc) {
+  std::atomic<bool> ok{true};
+  host_rows(n, [&](int64_t lo, int64_t hi) {
+    for (int64_t r = lo; r < hi && ok.load(std::memory_order_relaxed); ++r)
+      for (int32_t j = hp[r]; j < hp[r + 1]; ++j)
+        if (std::llabs((int64_t)hc[j] - r) > 32767) {
+          ok = false;
+          break;
+        }
+  });
+  return ok;
+}
+
+void build_col16(ks_mat* m) {
+  ks_ctx* c = m->ctx;
+  m->col16.alloc(m->nnz);
+  k_col_offsets<<<grid_map(m->n), kBlock, 0, c->stream>>>(m->n, m->ptr.p, m->col.p, m->col16.p);
+  KS_HIPCK(hipGetLastError());
+  if (m->has_t) {
+    m->tcol16.alloc(m->nnz);
+    k_col_offsets<<<grid_map(m->n), kBlock, 0, c->stream>>>(m->n, m->tptr.p, m->tcol.p, m->tcol16.p);
+    KS_HIPCK(hipGetLastError());
+  }
 }
 
 // Split the CSR (hp, hc: host row pointers and columns of A or A^T) into
@@ -826,11 +856,13 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   for (int64_t j = 0; j < nnz; ++j)
     KS_CHECK(hc[j] >= 0 && hc[j] < n, KS_ERR_BAD_INPUT, "column index out of range at entry " + std::to_string(j));
   m->max_row = mx;
+  const bool off16 = !(flags & KS_MAT_COL32) && nnz > 0 && offsets_fit16(n, hp.data(), hc.data());
   make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0, tile_choice(flags),
               merge_choice(flags, n, nnz, mx));
   if (nnz > 0 && panel_choice(flags, n, hp.data(), hc.data()))
     build_panels(m.get(), m->panels, hp.data(), hc.data(), nullptr, flags);
   if (flags & KS_MAT_TRANSPOSE) build_transpose(m.get());
+  if (off16) build_col16(m.get());
   if (flags & KS_MAT_FP32_COPY) ensure_fp32(m.get());
   KS_HIPCK(hipStreamSynchronize(c->stream));
   *out = m.release();
@@ -900,6 +932,13 @@ ks_status ks_mat_info(const ks_mat* m, int64_t* n, int64_t* nnz, int32_t* kind, 
   if (nnz) *nnz = m->nnz;
   if (kind) *kind = m->panels.np ? 3 : m->plan.kind;
   if (lanes) *lanes = m->plan.L;
+  return KS_OK;
+}
+
+ks_status ks_mat_index_bytes(const ks_mat* m, int32_t trans, int32_t precision, int32_t* bytes) {
+  if (!m || !bytes || (precision != KS_F64 && precision != KS_F32)) return KS_ERR_BAD_INPUT;
+  if (trans && !m->has_t) return KS_ERR_BAD_INPUT;
+  *bytes = precision == KS_F64 ? m->view<double>(trans != 0).ib : m->view<float>(trans != 0).ib;
   return KS_OK;
 }
 

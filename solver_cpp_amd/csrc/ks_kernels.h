@@ -387,14 +387,15 @@ struct TileArr {
 // One thread's share of a tile: values and column indices of the window
 // [a0, a0 + WIN), a0 = j0 & ~3, plus the row pointer of row r0 + t.  Chunks
 // are 16 bytes of values (4 FP32 or 2 FP64 entries) and as many column
-// indices (16 or 8 bytes).
-template <class T, int NTH, int WIN>
+// indices: int32 columns (16 or 8 bytes) or int16 offsets col - row (8 or 4
+// bytes; see k_spmv_stream).
+template <class T, int NTH, int WIN, class IX = int32_t>
 struct QuadTile {
   static constexpr int W = 16 / (int)sizeof(T);
   TileArr<T, NTH, WIN, W> v;
-  TileArr<int32_t, NTH, WIN, W> c;
+  TileArr<IX, NTH, WIN, W> c;
   int32_t pr;
-  __device__ __forceinline__ void load(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+  __device__ __forceinline__ void load(const int32_t* __restrict__ ptr, const IX* __restrict__ col,
                                        const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1,
                                        int32_t nnz, int t) {
     if (j1 - j0 > WIN - 3) return;  // long row: loaded in place
@@ -408,14 +409,14 @@ struct QuadTile {
     }
     pr = t < r1 - r0 ? ptr[r0 + t] : 0;
   }
-  __device__ __forceinline__ void stage(T* sv, int32_t* sc, int32_t j0, int32_t j1, int t) const {
+  __device__ __forceinline__ void stage(T* sv, IX* sc, int32_t j0, int32_t j1, int t) const {
     const int32_t span = j1 - (j0 & ~3);
     v.stage(sv, span, t);
     c.stage(sc, span, t);
   }
 };
-template <class T>
-using TileData = QuadTile<T, kBlock, kTileWin>;
+template <class T, class IX = int32_t>
+using TileData = QuadTile<T, kBlock, kTileWin, IX>;
 
 // Register budget of the CSR-stream kernels as waves/SIMD: FP32 64 VGPRs
 // (8 waves) for a plain SpMV and 72 (7 waves) with a fused reduction, FP64
@@ -436,18 +437,23 @@ struct HasProj : std::false_type {};
 template <class E>
 struct HasProj<E, std::void_t<decltype(E::kProj)>> : std::true_type {};
 
-template <class T, int NX, int NV, class Epi>
+// IX = int16_t: the column index arrives as the offset col - row (every
+// offset of the matrix in [-32767, 32767]; stencil and other banded
+// matrices), half the index bytes of int32 columns.  The row sums are the
+// same: entry for entry, the same x is gathered in the same order.
+template <class T, int NX, int NV, class Epi, class IX = int32_t>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves<T, NX, NV>))) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
                                                         const int32_t* __restrict__ tnz,
                                                         const int32_t* __restrict__ ptr,
-                                                        const int32_t* __restrict__ col,
+                                                        const IX* __restrict__ col,
                                                         const T* __restrict__ val, const T* __restrict__ x0,
                                                         const T* __restrict__ x1, Epi epi, Gate g, RedArgs ra,
                                                         StatArgs st) {
   if (g.off()) return;
   st.count();
+  constexpr bool kRel = sizeof(IX) == 2;
   __shared__ __align__(16) T sv[kTileWin];
-  __shared__ __align__(16) int32_t sc[kTileWin];
+  __shared__ __align__(16) IX sc[kTileWin];
   __shared__ int32_t sptr[kTileRows + 1];
   __shared__ double slong[NX][kBlock / 64];
   double acc[NV > 0 ? NV : 1];
@@ -459,7 +465,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
   rbs.range(t, end, step);
   const int32_t nnz = tnz[ntiles];
   int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
-  TileData<T> d;
+  TileData<T, IX> d;
   if (t < end) {
     r0 = tiles[t]; r1 = tiles[t + 1]; j0 = tnz[t]; j1 = tnz[t + 1];
     d.load(ptr, col, val, r0, r1, j0, j1, nnz, tid);
@@ -483,7 +489,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
       T s0 = 0, s1 = 0;
       for (int k = tid; k < cnt; k += kBlock) {
         const T a = ld_stream(val + j0 + k);
-        const int c = ld_stream(col + j0 + k);
+        const int c = (int)ld_stream(col + j0 + k) + (kRel ? r0 : 0);
         s0 += a * x0[c];
         if (NX == 2) s1 += a * x1[c];
       }
@@ -523,11 +529,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
       const int grp = tid / tpr, lane = tid % tpr;
       if (grp < nr) {
         const int s = sptr[grp], e = sptr[grp + 1];
+        const int cb = kRel ? r0 + grp : 0;
         T a0 = 0, a1 = 0;
 #pragma unroll 2
         for (int k = s + lane; k < e; k += tpr) {
           const T v = sv[k];
-          const int c = sc[k];
+          const int c = cb + (int)sc[k];
           a0 = mul_add_rn(a0, v, x0[c]);
           if (NX == 2) a1 = mul_add_rn(a1, v, x1[c]);
         }
@@ -1582,6 +1589,15 @@ __global__ void k_gather(int64_t m, const int32_t* __restrict__ perm, const Tin*
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride)
     out[j] = (Tout)in[perm[j]];
+}
+
+// out[j] = col[j] - r for every entry j of row r (the int16 column offsets
+// of a matrix whose offsets all lie in [-32767, 32767], checked by the host)
+__global__ void k_col_offsets(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                              int16_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride)
+    for (int32_t j = ptr[r], e = ptr[r + 1]; j < e; ++j) out[j] = (int16_t)(col[j] - r);
 }
 
 // out[0..m) = v: clears memory with a kernel (no memset node), for state

@@ -760,16 +760,24 @@ void launch_spmv_proj(ks_ctx* c, const MatView<T>& A, const T* vk, T* w, const d
                       int nv, double* hcol, int ldh, Gate g, const double* vn, PendNorm pn = PendNorm{}) {
   if (spmv_proj_fused(A, V, ldv, nv, (const T*)w)) {
     const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
-    const double bytes = spmv_bytes<T>(A.n, A.nnz) + (double)sizeof(T) * A.n * nv;
+    const double bytes = spmv_bytes<T>(A.n, A.nnz, A.ib) + (double)sizeof(T) * A.n * nv;
     EpiDivProj<T> e{};
     e.y = w; e.r = r; e.V = V; e.ldv = ldv; e.nv = nv;
     if (pn.pk >= 0) {
       e.np = c->part2.p; e.ng = pn.ng;
     }
-    auto kern = k_spmv_stream<T, 1, 0, EpiDivProj<T>>;
-    const int G = grid_rows(c, kern, A.ntiles);
-    klaunch(c, cls, kern, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, vk, vk, e, g, red(c),
-            stat(c, cls, bytes));
+    int G;
+    if (A.col16) {
+      auto kern = k_spmv_stream<T, 1, 0, EpiDivProj<T>, int16_t>;
+      G = grid_rows(c, kern, A.ntiles);
+      klaunch(c, cls, kern, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col16, A.val, vk, vk, e, g, red(c),
+              stat(c, cls, bytes));
+    } else {
+      auto kern = k_spmv_stream<T, 1, 0, EpiDivProj<T>>;
+      G = grid_rows(c, kern, A.ntiles);
+      klaunch(c, cls, kern, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, vk, vk, e, g, red(c),
+              stat(c, cls, bytes));
+    }
     if (pn.pk >= 0) launch_proj_finish(c, G, nv, FinNormProj{FinProj{hcol, ldh, nv, vn}, pn.c, pn.pk,
                                                              c->part2.p, pn.ng, 0.0}, g);
     else launch_proj_finish(c, G, nv, FinProj{hcol, ldh, nv, vn}, g);

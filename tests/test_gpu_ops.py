@@ -21,7 +21,7 @@ def _mat(ctx, ptr, col, val, **kw):
     return api.Matrix(ctx, ptr, col, val, **kw)
 
 
-LAYOUTS = ["auto", "stream", "wg", "wave", "vector", "merge", "panels"]
+LAYOUTS = ["auto", "col32", "stream", "wg", "wg_col32", "wave", "vector", "merge", "panels"]
 
 
 def _auto_kind(ptr):
@@ -42,7 +42,8 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     n = len(ptr) - 1
     x = np.random.default_rng(7).standard_normal(n)
     A = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
-    kinds = {"auto": _auto_kind(ptr), "stream": 0, "wg": 0, "wave": 0, "vector": 1, "merge": 2, "panels": 3}
+    kinds = {"auto": _auto_kind(ptr), "col32": _auto_kind(ptr), "stream": 0, "wg": 0, "wg_col32": 0, "wave": 0,
+             "vector": 1, "merge": 2, "panels": 3}
     assert A.info()["spmv_kind"] == kinds[layout]
     m = np.diff(ptr).max()
     y = A.spmv(x)
@@ -68,7 +69,7 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
 
 
-@pytest.mark.parametrize("layout", ["stream", "wg", "wave"])
+@pytest.mark.parametrize("layout", ["stream", "wg", "wg_col32", "wave"])
 @pytest.mark.parametrize("case", ["poisson2d", "convdiff3d", "aniso"])
 def test_spmv_bitexact_short_rows(ctx, oracle, case, layout):
     # rows of <= 8 entries are summed sequentially in CSR order from products
@@ -85,6 +86,56 @@ def test_spmv_bitexact_short_rows(ctx, oracle, case, layout):
     assert np.array_equal(A.spmv(x), oracle.spmv(ptr, col, val, x))
     x32 = x.astype(np.float32)
     assert np.array_equal(A.spmv(x32, precision=32), oracle.spmv_f32(ptr, col, val.astype(np.float32), x32))
+
+
+def _band_csr(n, offsets, seed):
+    # row i holds the entries i + d, d in offsets, that fall inside [0, n)
+    rows = [[i + d for d in sorted(offsets) if 0 <= i + d < n] for i in range(n)]
+    ptr = np.zeros(n + 1, np.int32)
+    ptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([c for r in rows for c in r], np.int32)
+    val = np.random.default_rng(seed).standard_normal(len(col))
+    return ptr, col, val
+
+
+@pytest.mark.parametrize("reach", [32767, 32768])
+def test_spmv_col16_offset_limit(ctx, oracle, reach):
+    # entries at row -+ reach: within int16 offsets at 32767 (the SpMV then
+    # reads col - row as int16, A^T too), int32 columns at 32768; both
+    # bit-exact against the oracle (rows of <= 8 entries) and equal to the
+    # int32 path
+    n = 70_000
+    ptr, col, val = _band_csr(n, [-reach, -3, 0, 1, reach], 21)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    B = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout="col32")
+    ib = 2 if reach <= 32767 else 4
+    for prec in (64, 32):
+        assert A.index_bytes(False, prec) == ib and A.index_bytes(True, prec) == ib
+        assert B.index_bytes(False, prec) == 4 and B.index_bytes(True, prec) == 4
+    x = np.random.default_rng(22).standard_normal(n)
+    y = A.spmv(x)
+    assert np.array_equal(y, oracle.spmv(ptr, col, val, x)) and np.array_equal(y, B.spmv(x))
+    otp, otc, otv = oracle.transpose(ptr, col, val)
+    yt = A.spmv(x, trans=True)
+    assert np.array_equal(yt, oracle.spmv(otp, otc, otv, x)) and np.array_equal(yt, B.spmv(x, trans=True))
+    x32 = x.astype(np.float32)
+    y32 = A.spmv(x32, precision=32)
+    assert np.array_equal(y32, oracle.spmv_f32(ptr, col, val.astype(np.float32), x32))
+    assert np.array_equal(y32, B.spmv(x32, precision=32))
+
+
+def test_spmv_col16_long_rows(ctx, oracle):
+    # rows longer than a workgroup tile (summed in place, not staged) read
+    # with int16 offsets: same sums as the int32 path
+    ptr, col, val, _ = P.zipf_small(20_000, 4096)
+    assert np.diff(ptr).max() > 2048
+    A = _mat(ctx, ptr, col, val, fp32=True, layout="wg")
+    B = _mat(ctx, ptr, col, val, fp32=True, layout="wg_col32")
+    assert A.index_bytes() == 2 and B.index_bytes() == 4
+    x = np.random.default_rng(23).standard_normal(len(ptr) - 1)
+    assert np.array_equal(A.spmv(x), B.spmv(x))
+    x32 = x.astype(np.float32)
+    assert np.array_equal(A.spmv(x32, precision=32), B.spmv(x32, precision=32))
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
