@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
 // [a0, a0 + window), so a tile holds at most window - 3 entries; a quad
 // that would pass nnz is read entry by entry, so no load passes nnz.
 constexpr int kTileWin = 2048;
-constexpr int kTileNnz = kTileWin - 3;
+constexpr int kTileNnz = kTileWin - 7;
 constexpr int kTileRows = 256;
 static_assert(kTileRows <= kBlock, "a workgroup tile row per thread");
 
@@ -392,14 +392,17 @@ struct TileArr {
 template <class T, int NTH, int WIN, class IX = int32_t>
 struct QuadTile {
   static constexpr int W = 16 / (int)sizeof(T);
+  static constexpr int WC = sizeof(IX) == 2 ? 8 : W;  // int16 offsets: 16-byte chunks of 8
+  static constexpr int kA = WC > 4 ? 8 : 4;           // window start alignment (entries)
+  __device__ static __forceinline__ int32_t a0of(int32_t j0) { return j0 & ~(kA - 1); }
   TileArr<T, NTH, WIN, W> v;
-  TileArr<IX, NTH, WIN, W> c;
+  TileArr<IX, NTH, WIN, WC> c;
   int32_t pr;
   __device__ __forceinline__ void load(const int32_t* __restrict__ ptr, const IX* __restrict__ col,
                                        const T* __restrict__ val, int32_t r0, int32_t r1, int32_t j0, int32_t j1,
                                        int32_t nnz, int t) {
-    if (j1 - j0 > WIN - 3) return;  // long row: loaded in place
-    const int32_t a0 = j0 & ~3;
+    if (j1 - j0 > WIN - kA + 1) return;  // long row: loaded in place
+    const int32_t a0 = a0of(j0);
     if (a0 + WIN <= nnz) {
       v.template load<true>(val, a0, j1, nnz, t);
       c.template load<true>(col, a0, j1, nnz, t);
@@ -410,7 +413,7 @@ struct QuadTile {
     pr = t < r1 - r0 ? ptr[r0 + t] : 0;
   }
   __device__ __forceinline__ void stage(T* sv, IX* sc, int32_t j0, int32_t j1, int t) const {
-    const int32_t span = j1 - (j0 & ~3);
+    const int32_t span = j1 - a0of(j0);
     v.stage(sv, span, t);
     c.stage(sc, span, t);
   }
@@ -516,8 +519,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       __syncthreads();
     } else {
-      if (tid < nr) sptr[tid] = d.pr - (j0 & ~3);
-      if (tid == 0) sptr[nr] = j1 - (j0 & ~3);
+      if (tid < nr) sptr[tid] = d.pr - d.a0of(j0);
+      if (tid == 0) sptr[nr] = j1 - d.a0of(j0);
       d.stage(sv, sc, j0, j1, tid);
       __syncthreads();
       // in flight during the sums: the next tile's data, then the meta after it
