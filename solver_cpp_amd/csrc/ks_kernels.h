@@ -558,73 +558,113 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
     // dots (y, V_v) over this workgroup's own rows, which it has just
     // written (visible to the whole workgroup after the loop's last
     // barrier; still in this XCD's L2), so y is not read back from HBM and
-    // no separate projection launch runs.  The columns are split among the
-    // wavefronts (wavefront w takes columns w, w + 4, ..., four at a time)
-    // and each walks all of the workgroup's tiles, the next tile's bounds
-    // loaded while the current one is summed; rows are read in 16-byte
-    // chunks (y, V and ldv 16-byte aligned, checked by the host), a tile's
-    // rows outside its aligned chunks one per lane.  Each wavefront sums its
-    // columns over its lanes and writes them to part[v * kMaxGrid +
-    // workgroup] itself, so no barrier follows.
+    // no separate projection launch runs.  The four wavefronts split the
+    // work by columns and by tiles: gc column groups of cpg = nv / gc
+    // columns (gc = 1, 2 or 4, the fewest with cpg <= CP when possible),
+    // each taken by 4 / gc wavefronts that split the workgroup's tiles
+    // among them (every (4/gc)-th tile), CP columns per pass, with the next
+    // tile's bounds loaded while the current one is summed.  So every
+    // wavefront has work at every column count, and a tile costs one round
+    // trip per pass.  Rows are read in 16-byte chunks (y, V and ldv 16-byte
+    // aligned, checked by the host; a tile of <= 256 rows is at most NCH
+    // chunks per lane), a tile's rows outside its aligned chunks one per
+    // lane.  Per pass the sums of a column's wavefronts are added in a
+    // fixed order through LDS and written to part[v * kMaxGrid + workgroup].
     static_assert(NV == 0 && NX == 1, "projection epilogue takes a plain SpMV");
     using VT = typename Chunk<T, 16 / sizeof(T)>::type;
     constexpr int W = 16 / sizeof(T);
     constexpr int kW = kBlock / 64;
-    constexpr int CP = 4;  // columns per wavefront per pass
-    const int lane = tid & 63, wv = tid >> 6;
+    static_assert(kW == 4, "four wavefronts per workgroup");
+    constexpr int CP = sizeof(T) == 4 ? 6 : 4;   // columns per pass (the most without spills)
+    constexpr int NCH = kTileRows / (64 * W);     // aligned chunks per lane per tile
+    __shared__ double sred[kW][CP];
+    // the wavefront index is uniform across the wavefront: scalar registers
+    // for the tile loop and its bounds
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     int64_t u0, uend, ustep;
     rbs.range(u0, uend, ustep);
     const int nv = epi.nv;
     const int64_t ldv = epi.ldv;
-    for (int c0 = wv; c0 < nv; c0 += kW * CP) {
+    const int gc = nv <= CP ? 1 : nv <= 2 * CP ? 2 : 4;
+    const int gt = kW / gc, cpg = (nv + gc - 1) / gc;
+    const int cbeg = (wv / gt) * cpg, cend = min(nv, cbeg + cpg);
+    const int64_t ws = (int64_t)gt * ustep;
+    for (int p0 = 0; p0 < cpg; p0 += CP) {
+      const int c0 = cbeg + p0;  // columns c0 + j < cend in this pass
       double pa[CP];
 #pragma unroll
       for (int j = 0; j < CP; ++j) pa[j] = 0.0;
       const T* vb = epi.V + (int64_t)c0 * ldv;
+      int64_t u = u0 + (int64_t)(wv % gt) * ustep;
       int64_t q0 = 0, q1 = 0;
-      if (u0 < uend) { q0 = tiles[u0]; q1 = tiles[u0 + 1]; }
-      for (int64_t u = u0; u < uend; u += ustep) {
+      if (u < uend) { q0 = tiles[u]; q1 = tiles[u + 1]; }
+      for (; u < uend; u += ws) {
         int64_t n0 = 0, n1 = 0;
-        if (u + ustep < uend) { n0 = tiles[u + ustep]; n1 = tiles[u + ustep + 1]; }
+        if (u + ws < uend) { n0 = tiles[u + ws]; n1 = tiles[u + ws + 1]; }
         const int64_t a = (q0 + W - 1) / W * W, b = q1 / W * W;
-        for (int64_t r = a + (int64_t)W * lane; r + W <= b; r += 64 * W) {
-          const VT yv = *reinterpret_cast<const VT*>(epi.y + r);
-          VT xv[CP];
+        {
+          VT yv[NCH], xv[NCH][CP];
 #pragma unroll
-          for (int j = 0; j < CP; ++j)
-            if (c0 + kW * j < nv) xv[j] = *reinterpret_cast<const VT*>(vb + (int64_t)(kW * j) * ldv + r);
+          for (int h = 0; h < NCH; ++h) {
+            const int64_t r = a + (int64_t)W * (lane + 64 * h);
+            if (r + W <= b) {
+              yv[h] = *reinterpret_cast<const VT*>(epi.y + r);
 #pragma unroll
-          for (int j = 0; j < CP; ++j)
-            if (c0 + kW * j < nv) {
-              double t = 0.0;
-#pragma unroll
-              for (int e = 0; e < W; ++e) t += (double)yv[e] * (double)xv[j][e];
-              pa[j] += t;
+              for (int j = 0; j < CP; ++j)
+                if (c0 + j < cend) xv[h][j] = *reinterpret_cast<const VT*>(vb + (int64_t)j * ldv + r);
             }
+          }
+#pragma unroll
+          for (int h = 0; h < NCH; ++h) {
+            const int64_t r = a + (int64_t)W * (lane + 64 * h);
+            if (r + W <= b) {
+#pragma unroll
+              for (int j = 0; j < CP; ++j)
+                if (c0 + j < cend) {
+                  double t = 0.0;
+#pragma unroll
+                  for (int e = 0; e < W; ++e) t += (double)yv[h][e] * (double)xv[h][j][e];
+                  pa[j] += t;
+                }
+            }
+          }
         }
-        // rows before the first and after the last aligned chunk (< 2W)
+        // rows before the first and after the last aligned chunk (< 2W; none
+        // on tiles with chunk-aligned bounds), after the chunks
         const int64_t hn = (a < q1 ? a : q1) - q0;
         const int64_t tb = b > a ? b : a;
         int64_t row = -1;
         if (lane < hn) row = q0 + lane;
         else if (lane >= W && lane - W < q1 - tb) row = tb + (lane - W);
         if (row >= 0) {
-          const double wr = (double)epi.y[row];
+          const double ye = (double)epi.y[row];
 #pragma unroll
           for (int j = 0; j < CP; ++j)
-            if (c0 + kW * j < nv) pa[j] += wr * (double)vb[(int64_t)(kW * j) * ldv + row];
+            if (c0 + j < cend) pa[j] += ye * (double)vb[(int64_t)j * ldv + row];
         }
         q0 = n0; q1 = n1;
       }
 #pragma unroll
       for (int j = 0; j < CP; ++j) {
         const double v = wave_sum(pa[j]);
-        if (lane == 0 && c0 + kW * j < nv) ra.part[(size_t)(c0 + kW * j) * kMaxGrid + blockIdx.x] = v;
+        if (lane == 0) sred[wv][j] = v;
       }
+      __syncthreads();
+      if (tid < gc * CP) {
+        const int g = tid / CP, j = tid % CP, c = g * cpg + p0 + j;
+        if (p0 + j < cpg && c < nv) {
+          const double* q = &sred[g * gt][j];
+          double t;
+          if (gt == 4) t = (q[0] + q[CP]) + (q[2 * CP] + q[3 * CP]);
+          else if (gt == 2) t = q[0] + q[CP];
+          else t = q[0];
+          ra.part[(size_t)c * kMaxGrid + blockIdx.x] = t;
+        }
+      }
+      __syncthreads();
     }
   }
 }
-
 
 // Wavefront CSR-stream: the scheme of k_spmv_stream (values and column
 // indices staged in LDS, one row per lane for short rows) with every wavefront
