@@ -395,14 +395,18 @@ template <class T, int NX, int NV, class Epi>
 int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
                 double bytes) {
   int G;
-  if (A.wg_tiles && A.col16) {
-    G = grid_rows(c, k_spmv_stream<T, NX, NV, Epi, int16_t>, A.ntiles);
-    klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi, int16_t>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col16,
-            A.val, x0, x1, epi, g, red(c), stat(c, cls, bytes));
-  } else if (A.wg_tiles) {
+  if (A.wg_tiles) {
+    // the grid (and so the reduction partials and their summation order) is
+    // that of the int32 kernel in both index formats, so a solve's results do
+    // not depend on the format: the int16 kernel, which needs fewer
+    // registers, may have room for more workgroups than it is given
     G = grid_rows(c, k_spmv_stream<T, NX, NV, Epi>, A.ntiles);
-    klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
-            epi, g, red(c), stat(c, cls, bytes));
+    if (A.col16)
+      klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi, int16_t>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col16,
+              A.val, x0, x1, epi, g, red(c), stat(c, cls, bytes));
+    else
+      klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0,
+              x1, epi, g, red(c), stat(c, cls, bytes));
   } else {
     G = grid_rows(c, k_spmv_wave<T, NX, NV, Epi>, (A.ntiles + kBlock / 64 - 1) / (kBlock / 64));
     klaunch(c, cls, k_spmv_wave<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
