@@ -627,7 +627,8 @@ struct EpiDiv {
       inv = __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
     }
   }
-  __device__ void row(int64_t i, T s, T, double*) { y[i] = s * inv; }
+  __device__ T value(T s) const { return s * inv; }
+  __device__ void row(int64_t i, T s, T, double*) { y[i] = value(s); }
   __device__ void fin(const double*) const {}
 };
 // sqrt of the sum of the ng <= kMaxGrid partials p[] (a norm whose update

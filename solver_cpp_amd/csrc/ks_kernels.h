@@ -455,8 +455,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
   if (g.off()) return;
   st.count();
   constexpr bool kRel = sizeof(IX) == 2;
+  // With the projection epilogue, the y of the workgroup's first kYT tiles
+  // is kept in LDS (tile s at sy[s][row - (r0 & ~3)], 16-byte aligned row
+  // chunks): the projection reads it there instead of from L2/HBM, and it
+  // goes to HBM in one burst per tile during the projection rather than
+  // between the matrix reads.  kYT is what fits beside the tile buffers at
+  // the kernel's occupancy (7 FP32 / 5 FP64 workgroups per CU in 160 KB).
+  constexpr int kYT = !HasProj<Epi>::value ? 0 : sizeof(T) == 4 ? (kRel ? 9 : 4) : (kRel ? 4 : 2);
+  constexpr int kYW = kTileRows + 4;
   __shared__ __align__(16) T sv[kTileWin];
   __shared__ __align__(16) IX sc[kTileWin];
+  __shared__ __align__(16) T sy[kYT > 0 ? kYT : 1][kYT > 0 ? kYW : 1];
   __shared__ int32_t sptr[kTileRows + 1];
   __shared__ double slong[NX][kBlock / 64];
   double acc[NV > 0 ? NV : 1];
@@ -482,7 +491,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
   // the epilogue's scalars, with the first tile already in flight (an
   // epilogue may reduce partials across the workgroup here)
   epi.prep();
-  for (; t < end; t += step) {
+  int slot = 0;  // this tile's index among the workgroup's tiles
+  for (; t < end; t += step, ++slot) {
     const int nr = r1 - r0, cnt = j1 - j0;
     const int64_t tn = t + step, tnn = t + 2 * step;
     const bool more = tn < end, more2 = tnn < end;
@@ -513,7 +523,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
           a0 += (T)slong[0][k];
           if (NX == 2) a1 += (T)slong[NX - 1][k];
         }
-        epi.row(r0, a0, a1, acc);
+        if constexpr (kYT > 0) {
+          if (slot < kYT) sy[slot][r0 & 3] = epi.value(a0);
+          else epi.row(r0, a0, a1, acc);
+        } else {
+          epi.row(r0, a0, a1, acc);
+        }
       }
       if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, nnz, tid);
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
@@ -545,7 +560,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
           a0 += __shfl_xor(a0, off, 64);
           if (NX == 2) a1 += __shfl_xor(a1, off, 64);
         }
-        if (lane == 0) epi.row(r0 + grp, a0, a1, acc);
+        if (lane == 0) {
+          if constexpr (kYT > 0) {
+            if (slot < kYT) sy[slot][(r0 & 3) + grp] = epi.value(a0);
+            else epi.row(r0 + grp, a0, a1, acc);
+          } else {
+            epi.row(r0 + grp, a0, a1, acc);
+          }
+        }
       }
       __syncthreads();
     }
@@ -589,26 +611,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
     const int gt = kW / gc, cpg = (nv + gc - 1) / gc;
     const int cbeg = (wv / gt) * cpg, cend = min(nv, cbeg + cpg);
     const int64_t ws = (int64_t)gt * ustep;
-    for (int p0 = 0; p0 < cpg; p0 += CP) {
+    // at least one pass: its column group 0 also writes the staged y out
+    for (int p0 = 0; p0 == 0 || p0 < cpg; p0 += CP) {
       const int c0 = cbeg + p0;  // columns c0 + j < cend in this pass
       double pa[CP];
 #pragma unroll
       for (int j = 0; j < CP; ++j) pa[j] = 0.0;
       const T* vb = epi.V + (int64_t)c0 * ldv;
       int64_t u = u0 + (int64_t)(wv % gt) * ustep;
+      int ti = wv % gt;  // the tile's index among the workgroup's tiles
       int64_t q0 = 0, q1 = 0;
       if (u < uend) { q0 = tiles[u]; q1 = tiles[u + 1]; }
-      for (; u < uend; u += ws) {
+      // one tile: y from LDS (a staged tile, kStaged) or from memory; the
+      // staged tiles come first in every wavefront's order, so two loops
+      // keep each load in one address space
+      auto tile = [&](auto staged_tag) {
+        constexpr bool kStaged = decltype(staged_tag)::value;
         int64_t n0 = 0, n1 = 0;
         if (u + ws < uend) { n0 = tiles[u + ws]; n1 = tiles[u + ws + 1]; }
         const int64_t a = (q0 + W - 1) / W * W, b = q1 / W * W;
+        const int64_t base = q0 & ~(int64_t)3;
+        const T* ys = nullptr;  // row r of a staged tile at ys[r - base]
+        if constexpr (kStaged) {
+          ys = sy[ti];
+          if (p0 == 0 && wv < gt)
+            for (int64_t j = q0 + lane; j < q1; j += 64) epi.y[j] = ys[j - base];
+        }
         {
           VT yv[NCH], xv[NCH][CP];
 #pragma unroll
           for (int h = 0; h < NCH; ++h) {
             const int64_t r = a + (int64_t)W * (lane + 64 * h);
             if (r + W <= b) {
-              yv[h] = *reinterpret_cast<const VT*>(epi.y + r);
+              if constexpr (kStaged) yv[h] = *reinterpret_cast<const VT*>(ys + (r - base));
+              else yv[h] = *reinterpret_cast<const VT*>(epi.y + r);
 #pragma unroll
               for (int j = 0; j < CP; ++j)
                 if (c0 + j < cend) xv[h][j] = *reinterpret_cast<const VT*>(vb + (int64_t)j * ldv + r);
@@ -637,13 +673,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
         if (lane < hn) row = q0 + lane;
         else if (lane >= W && lane - W < q1 - tb) row = tb + (lane - W);
         if (row >= 0) {
-          const double ye = (double)epi.y[row];
+          double ye;
+          if constexpr (kStaged) ye = (double)ys[row - base];
+          else ye = (double)epi.y[row];
 #pragma unroll
           for (int j = 0; j < CP; ++j)
             if (c0 + j < cend) pa[j] += ye * (double)vb[(int64_t)j * ldv + row];
         }
         q0 = n0; q1 = n1;
-      }
+      };
+      if constexpr (kYT > 0)
+        for (; u < uend && ti < kYT; u += ws, ti += gt) tile(std::true_type{});
+      for (; u < uend; u += ws, ti += gt) tile(std::false_type{});
 #pragma unroll
       for (int j = 0; j < CP; ++j) {
         const double v = wave_sum(pa[j]);
