@@ -66,7 +66,7 @@ typedef enum ks_status {
                                longest row exceeds 256 entries and 16x the mean) */
 #define KS_MAT_VECTOR   64u /* force the lanes-per-row vector SpMV */
 /* CSR-stream tiles: by default the SpMV works on workgroup tiles (256 rows /
- * 2045 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 509) */
+ * 2041 entries); KS_MAT_WAVE_TILES selects wavefront tiles (64 / 509) */
 #define KS_MAT_WG_TILES 128u   /* workgroup tiles (the default) */
 #define KS_MAT_WAVE_TILES 512u /* wavefront tiles */
 #define KS_MAT_PANELS   1024u /* column panels: the SpMV runs as 10 passes over

@@ -302,12 +302,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const int32_t* __res
 // indices of the workgroup's next tile are already in flight (eight
 // independent loads per thread).  NX = 2 forms A x0 and A x1 from one pass
 // over the matrix (the k-skip matrix-powers basis).
-// Values and column indices are read as quads (entries e..e+3 with e a
-// multiple of 4: 16-byte column loads, 16- or 2x16-byte value loads) and
-// staged in LDS at their offsets from the quad-aligned tile start a0 =
-// j0 & ~3 with aligned 16-byte LDS stores.  A tile's quads cover
-// [a0, a0 + window), so a tile holds at most window - 3 entries; a quad
-// that would pass nnz is read entry by entry, so no load passes nnz.
+// Values and column indices are read in aligned chunks (16 bytes of values;
+// int32 columns in chunks of as many entries, int16 offsets in 16-byte
+// chunks of 8) and staged in LDS at their offsets from the aligned tile
+// start a0 (j0 & ~3, or j0 & ~7 with int16 offsets) with aligned LDS stores.
+// The chunks cover [a0, a0 + window), so a workgroup tile holds at most
+// window - 7 entries; a chunk that would pass nnz is read entry by entry, so
+// no load passes nnz.
 constexpr int kTileWin = 2048;
 constexpr int kTileNnz = kTileWin - 7;
 constexpr int kTileRows = 256;
@@ -385,10 +386,10 @@ struct TileArr {
 };
 
 // One thread's share of a tile: values and column indices of the window
-// [a0, a0 + WIN), a0 = j0 & ~3, plus the row pointer of row r0 + t.  Chunks
-// are 16 bytes of values (4 FP32 or 2 FP64 entries) and as many column
-// indices: int32 columns (16 or 8 bytes) or int16 offsets col - row (8 or 4
-// bytes; see k_spmv_stream).
+// [a0, a0 + WIN), a0 = a0of(j0), plus the row pointer of row r0 + t.  Chunks
+// are 16 bytes of values (4 FP32 or 2 FP64 entries) and column indices:
+// int32 columns, as many as the value chunk (16 or 8 bytes), or int16
+// offsets col - row, 8 to a 16-byte chunk (see k_spmv_stream).
 template <class T, int NTH, int WIN, class IX = int32_t>
 struct QuadTile {
   static constexpr int W = 16 / (int)sizeof(T);
