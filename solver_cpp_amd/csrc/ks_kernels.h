@@ -1203,10 +1203,12 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
 // when the cycle stopped inside); y and size come from the Ctl (k_hye).
 // Tx: type of x, Tb: type of the basis B.
 // With c->vn set (and use_vn), B holds the unnormalised basis: y_i / vn[i].
+// With xw set, the new x is also written there widened to FP64 (an FP32
+// solve's output for the level above it).
 template <class Tx, class Tb>
 __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __restrict__ B, int64_t ldb,
-                                                     Tx* __restrict__ x, const Ctl* c, int last_term,
-                                                     int use_vn, Gate g, StatArgs st) {
+                                                     Tx* __restrict__ x, double* __restrict__ xw, const Ctl* c,
+                                                     int last_term, int use_vn, Gate g, StatArgs st) {
   if (g.off()) return;
   st.count();
   extern __shared__ double sy[];
@@ -1223,14 +1225,18 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __rest
   if (s_lt) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
       double t = sz > 0 ? sy[sz - 1] * (double)B[(int64_t)(sz - 1) * ldb + i] : 0.0;
-      x[i] = (Tx)((double)x[i] + (double)(Tb)t);
+      const Tx xi = (Tx)((double)x[i] + (double)(Tb)t);
+      x[i] = xi;
+      if (xw) xw[i] = (double)xi;
     }
   } else {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
       double t = 0.0;
 #pragma unroll 8
       for (int v = 0; v < sz; ++v) t += sy[v] * (double)B[(int64_t)v * ldb + i];
-      x[i] = (Tx)((double)x[i] + (double)(Tb)t);
+      const Tx xi = (Tx)((double)x[i] + (double)(Tb)t);
+      x[i] = xi;
+      if (xw) xw[i] = (double)xi;
     }
   }
 }

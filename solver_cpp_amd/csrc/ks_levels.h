@@ -909,14 +909,20 @@ struct GmresLevel : Level {
     k_hye<<<1, 1, 0, c->stream>>>(h, gdone());
     KS_HIPCK(hipGetLastError());
     const int last = (cfg.compat & KS_COMPAT_LAST_TERM) ? 1 : 0;
-    const double bytes = (double)sizeof(T) * n * 2 + (inner ? 8.0 : (double)sizeof(T)) * n * (R - 1);
+    // an FP32 level's assembly also writes its FP64 output (xout), so no
+    // conversion pass follows: every change of x is an assembly, the last
+    // one included (enqueue_finish's cycle_end when the solve stopped at a
+    // cycle start)
+    double* xw = sizeof(T) != 8 ? xout : nullptr;
+    const double bytes = (double)sizeof(T) * n * 2 + (inner ? 8.0 : (double)sizeof(T)) * n * (R - 1) +
+                         (xw ? 8.0 * n : 0.0);
     const size_t lds = sizeof(double) * R;
     if (inner)
       klaunch(c, KS_K_CGS, k_assemble<T, double>, grid_map(c, k_assemble<T, double>, n, lds), lds, n, Z.p, ldv, x,
-              h, last, 0, gdone(), stat(c, KS_K_CGS, bytes));
+              (double*)nullptr, h, last, 0, gdone(), stat(c, KS_K_CGS, bytes));
     else
-      klaunch(c, KS_K_CGS, k_assemble<T, T>, grid_map(c, k_assemble<T, T>, n, lds), lds, n, V.p, ldv, x, h, last,
-              1, gdone(), stat(c, KS_K_CGS, bytes));
+      klaunch(c, KS_K_CGS, k_assemble<T, T>, grid_map(c, k_assemble<T, T>, n, lds), lds, n, V.p, ldv, x, xw, h,
+              last, 1, gdone(), stat(c, KS_K_CGS, bytes));
     k_mark_done<<<1, 1, 0, c->stream>>>(h, gdone());
     KS_HIPCK(hipGetLastError());
   }
@@ -926,10 +932,6 @@ struct GmresLevel : Level {
       pend = PendNorm{};
     }
     cycle_end();  // no-op unless the solve stopped in a cycle not yet assembled
-    if (sizeof(T) != 8) {
-      k_convert<T, double><<<grid_map(n), kBlock, 0, c->stream>>>(n, x, xout, gouter());
-      KS_HIPCK(hipGetLastError());
-    }
   }
   int64_t reported_iters(const Ctl& hc) const override { return hc.count + 1; }
   double unit_seconds_estimate() const override {
