@@ -273,6 +273,28 @@ struct OpGmFirstFin : OpGmFirst<T> {
   FinGmCycle f;
   __device__ void fin(const double* s) { f(s); }
 };
+// OpScale of the level above (v = w / vscale, FP64) fused with an inner
+// GMRES cycle start on v: r = (T)v, x = 0 and ||r||^2 in one pass - the same
+// values as the two passes.
+template <class T>
+struct OpGmFirstScaled {
+  const double* w;
+  const Ctl* sc;  // the level above: its vscale
+  double* v;
+  T *r, *x;
+  double vs;
+  FinGmCycle f;
+  __device__ void prep() { vs = sc->vscale; }
+  __device__ void elem(int64_t i, double* acc) {
+    const double vi = w[i] / vs;
+    v[i] = vi;
+    const T ri = (T)vi;
+    r[i] = ri;
+    if (x) x[i] = (T)0;
+    acc[0] += (double)ri * (double)ri;
+  }
+  __device__ void fin(const double* s) { f(s); }
+};
 struct FinGmNorm {
   Ctl* c;
   int k;
@@ -462,6 +484,13 @@ struct Level {
   // b (FP64, device) is the right-hand side; x_out (FP64, device) receives
   // the solution.  Inner levels always start from x = 0.
   virtual void enqueue_init(const double* b, double* x_out) = 0;
+  // Solve with the right-hand side v = w / sc->vscale, which this call also
+  // writes to v (the level above's basis column).  By default the scaling is
+  // a pass of its own; a level may fuse it into its first pass.
+  virtual void enqueue_init_scaled(const double* w, const Ctl* sc, double* v, double* x_out) {
+    launch_map<0>(c, n, OpScale<double>{w, v, sc, 0.0}, gouter(), KS_K_AXPY, 16.0 * n);
+    enqueue_init(v, x_out);
+  }
   virtual void enqueue_unit(int64_t u) = 0;
   virtual int64_t max_units() const = 0;
   virtual void enqueue_finish() {}
@@ -823,6 +852,9 @@ struct GmresLevel : Level {
   T* x = nullptr;
   double* xout = nullptr;
   const double* b = nullptr;
+  // enqueue_init_scaled: b = sw / ssc->vscale, formed by the first cycle start
+  const double* sw = nullptr;
+  const Ctl* ssc = nullptr;
   std::unique_ptr<Level> inner;
   void setup(ks_ctx* cx, const MatView<T>& a, const LevelCfg& cf, std::unique_ptr<Level> in = nullptr) {
     c = cx; A = a; n = a.n; cfg = cf; R = cf.restart;
@@ -857,7 +889,20 @@ struct GmresLevel : Level {
     xout = x_out;
     x = sizeof(T) == 8 ? (T*)x_out : xown.p;
     pend = PendNorm{};
+    sw = nullptr;
+    ssc = nullptr;
     if (x0given) launch_map<1>(c, n, OpSetBnorm{b, ctl()}, gouter(), KS_K_REDUCE, 8.0 * n);
+  }
+  // the scaling rides on the first cycle start (x0 = 0), which reads the
+  // right-hand side anyway
+  void enqueue_init_scaled(const double* w, const Ctl* sc, double* v, double* x_out) override {
+    if (x0given) {
+      Level::enqueue_init_scaled(w, sc, v, x_out);
+      return;
+    }
+    enqueue_init(v, x_out);
+    sw = w;
+    ssc = sc;
   }
   // Plain GMRES keeps its basis unnormalised: V_k holds w_k with norm
   // vn[k], the SpMV of step k divides its rows by vn[k], and the projection,
@@ -866,7 +911,11 @@ struct GmresLevel : Level {
   // inner solve takes v_k as a right-hand side.
   void cycle_start(int64_t cyc) {
     T* r0 = inner ? r.p : V.p;
-    if (cyc == 0 && !x0given) {
+    if (cyc == 0 && !x0given && sw) {
+      OpGmFirstScaled<T> op{sw, ssc, const_cast<double*>(b), r0, x, 0.0, FinGmCycle{ctl(), 1, 1}};
+      launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (16.0 + 2.0 * sizeof(T)) * n);
+      sw = nullptr;
+    } else if (cyc == 0 && !x0given) {
       OpGmFirstFin<T> op;
       op.src = b; op.r = r0; op.x = x; op.f = FinGmCycle{ctl(), 1, 1};
       launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (8.0 + 2.0 * sizeof(T)) * n);
@@ -874,7 +923,8 @@ struct GmresLevel : Level {
       Gate g = cyc == 0 ? gouter() : gstep();
       launch_spmv<T, 1>(c, A, x, EpiResid<T, FinGmCycle>{b, r0, FinGmCycle{ctl(), cyc == 0 ? 1 : 0, 0}}, g, 8.0 * n);
     }
-    if (inner) launch_map<0>(c, n, OpScale<T>{r.p, V.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
+    // VPGMRES: v_0 = r / ||r|| is formed by the inner solve's first pass
+    // (enqueue_init_scaled in enqueue_unit)
   }
   void enqueue_unit(int64_t u) override {
     const int64_t cyc = u / (R - 1);
@@ -884,13 +934,14 @@ struct GmresLevel : Level {
     T* vk1 = V.p + (int64_t)(k + 1) * ldv;
     Ctl* h = ctl();
     if (inner) {
+      // v_k = r / ||r|| (k = 0) or w / ||w|| (w from step k - 1), formed by
+      // the inner solve's first pass on it
       double* zk = Z.p + (int64_t)k * ldv;
-      inner->enqueue_init((const double*)vk, zk);
+      inner->enqueue_init_scaled(k == 0 ? (const double*)r.p : (const double*)w.p, h, (double*)vk, zk);
       for (int64_t iu = 0; iu < inner->max_units(); ++iu) inner->enqueue_unit(iu);
       inner->enqueue_finish();
       launch_spmv_proj<T>(c, A, (const T*)zk, w.p, nullptr, V.p, ldv, k + 1, hctl.H + k, hctl.ld, gstep(), nullptr);
       launch_cgs_update<T>(c, n, V.p, ldv, k + 1, w.p, w.p, hctl.H + k, hctl.ld, FinGmNorm{h, k}, gstep());
-      launch_map<0>(c, n, OpScale<T>{w.p, vk1, h, 0.0}, gstep(), KS_K_AXPY, 2.0 * sizeof(T) * n);
     } else {
       launch_spmv_proj<T>(c, A, vk, w.p, hctl.vr + k, V.p, ldv, k + 1, hctl.H + k, hctl.ld, gstep(), hctl.vn,
                           pend);
