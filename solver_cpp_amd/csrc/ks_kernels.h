@@ -55,7 +55,8 @@ struct Ctl {
   double *H, *e, *cs, *sn, *y;
   double* vn;      // GMRES: norms of the unnormalised basis vectors (R)
   double* vr;      // GMRES: 1 / vn (R), the SpMV epilogue's scale
-  int kskip, fix, compat, pad;
+  int kskip, fix, compat;
+  int xzero;       // GMRES: x is still the zero it started from (set at a cycle start)
   double *delta, *eta, *zeta, *rho, *phi;
 };
 
@@ -1204,7 +1205,9 @@ __global__ __launch_bounds__(kBlock) void k_cgs_update(int64_t n, const T* __res
 // Tx: type of x, Tb: type of the basis B.
 // With c->vn set (and use_vn), B holds the unnormalised basis: y_i / vn[i].
 // With xw set, the new x is also written there widened to FP64 (an FP32
-// solve's output for the level above it).
+// solve's output for the level above it).  While c->xzero is set (the first
+// cycle from x0 = 0) x is taken as 0.0 without being read, so the zero is
+// never written either.
 template <class Tx, class Tb>
 __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __restrict__ B, int64_t ldb,
                                                      Tx* __restrict__ x, double* __restrict__ xw, const Ctl* c,
@@ -1212,10 +1215,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __rest
   if (g.off()) return;
   st.count();
   extern __shared__ double sy[];
-  __shared__ int s_sz, s_lt;
+  __shared__ int s_sz, s_lt, s_xz;
   if (threadIdx.x == 0) {
     s_sz = c->stop ? c->ksz : c->R - 1;
     s_lt = last_term && c->stop;
+    s_xz = c->xzero;
   }
   __syncthreads();
   const int sz = s_sz;
@@ -1225,7 +1229,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __rest
   if (s_lt) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
       double t = sz > 0 ? sy[sz - 1] * (double)B[(int64_t)(sz - 1) * ldb + i] : 0.0;
-      const Tx xi = (Tx)((double)x[i] + (double)(Tb)t);
+      const Tx xi = (Tx)((s_xz ? 0.0 : (double)x[i]) + (double)(Tb)t);
       x[i] = xi;
       if (xw) xw[i] = (double)xi;
     }
@@ -1234,7 +1238,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t n, const Tb* __rest
       double t = 0.0;
 #pragma unroll 8
       for (int v = 0; v < sz; ++v) t += sy[v] * (double)B[(int64_t)v * ldb + i];
-      const Tx xi = (Tx)((double)x[i] + (double)(Tb)t);
+      const Tx xi = (Tx)((s_xz ? 0.0 : (double)x[i]) + (double)(Tb)t);
       x[i] = xi;
       if (xw) xw[i] = (double)xi;
     }

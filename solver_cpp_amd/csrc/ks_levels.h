@@ -240,7 +240,8 @@ struct OpScale {
   __device__ void fin(const double*) {}
 };
 
-// GMRES cycle start: r = (T)src (first cycle, x0 = 0) and the norm.
+// GMRES cycle start: r = (T)src (first cycle, x0 = 0) and the norm.  x is
+// not written: its finisher sets xzero, and the assembly takes x as zero.
 template <class T>
 struct OpGmFirst {
   const double* src;
@@ -249,16 +250,17 @@ struct OpGmFirst {
   __device__ void elem(int64_t i, double* acc) {
     const T ri = (T)src[i];
     r[i] = ri;
-    if (x) x[i] = (T)0;
     acc[0] += (double)ri * (double)ri;
   }
 };
+// xz: x is the zero the solve started from (the first cycle with x0 = 0)
 struct FinGmCycle {
   Ctl* c;
-  int first, set_bnorm;
+  int first, set_bnorm, xz;
   __device__ void operator()(const double* s) const {
     const double t = sqrt(s[0]);
     if (first) ctl_reset(c);
+    c->xzero = xz;
     if (set_bnorm) c->bnorm = t;
     for (int i = 0; i < c->R; ++i) c->e[i] = 0.0;
     c->e[0] = t;
@@ -274,8 +276,8 @@ struct OpGmFirstFin : OpGmFirst<T> {
   __device__ void fin(const double* s) { f(s); }
 };
 // OpScale of the level above (v = w / vscale, FP64) fused with an inner
-// GMRES cycle start on v: r = (T)v, x = 0 and ||r||^2 in one pass - the same
-// values as the two passes.
+// GMRES cycle start on v: r = (T)v and ||r||^2 in one pass (x = 0 as in
+// OpGmFirst) - the same values as the two passes.
 template <class T>
 struct OpGmFirstScaled {
   const double* w;
@@ -290,7 +292,6 @@ struct OpGmFirstScaled {
     v[i] = vi;
     const T ri = (T)vi;
     r[i] = ri;
-    if (x) x[i] = (T)0;
     acc[0] += (double)ri * (double)ri;
   }
   __device__ void fin(const double* s) { f(s); }
@@ -912,16 +913,17 @@ struct GmresLevel : Level {
   void cycle_start(int64_t cyc) {
     T* r0 = inner ? r.p : V.p;
     if (cyc == 0 && !x0given && sw) {
-      OpGmFirstScaled<T> op{sw, ssc, const_cast<double*>(b), r0, x, 0.0, FinGmCycle{ctl(), 1, 1}};
+      OpGmFirstScaled<T> op{sw, ssc, const_cast<double*>(b), r0, x, 0.0, FinGmCycle{ctl(), 1, 1, 1}};
       launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (16.0 + 2.0 * sizeof(T)) * n);
       sw = nullptr;
     } else if (cyc == 0 && !x0given) {
       OpGmFirstFin<T> op;
-      op.src = b; op.r = r0; op.x = x; op.f = FinGmCycle{ctl(), 1, 1};
+      op.src = b; op.r = r0; op.x = x; op.f = FinGmCycle{ctl(), 1, 1, 1};
       launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (8.0 + 2.0 * sizeof(T)) * n);
     } else {
       Gate g = cyc == 0 ? gouter() : gstep();
-      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinGmCycle>{b, r0, FinGmCycle{ctl(), cyc == 0 ? 1 : 0, 0}}, g, 8.0 * n);
+      launch_spmv<T, 1>(c, A, x, EpiResid<T, FinGmCycle>{b, r0, FinGmCycle{ctl(), cyc == 0 ? 1 : 0, 0, 0}}, g,
+                        8.0 * n);
     }
     // VPGMRES: v_0 = r / ||r|| is formed by the inner solve's first pass
     // (enqueue_init_scaled in enqueue_unit)
