@@ -10,6 +10,9 @@ generated inputs).
   (kskipcg.cpp:187-343) and CG.
 - 3-D 7-point Poisson 160^3: mixed VPGMRES(20) with ten FP32 GMRES inner
   steps (-R 21 -l 11 -r 12), to 1e-10 (vpgmres.cpp:746-975).
+- 3-D 7-point convection-diffusion 192^3, cell Peclet 0.8 (the adjoint
+  configuration's forward solve; its columns reach 36,864 rows, so the SpMV
+  reads int32 columns): the same mixed VPGMRES, to 1e-10.
 
 Bar: the same exit status, the reported iteration count within +-1 (k-skip:
 +-(k+1), since its residual is only checked once per block of k+1 steps),
@@ -46,6 +49,10 @@ CASES = {
     "vpgmres_mixed_poisson3d_160": ("poisson3d", "vpgmres", "gmres",
                                     dict(outer_restart=21, outer_maxloop=5000, outer_eps=1e-10, inner_maxloop=11,
                                          inner_restart=12, inner_eps=0.0, inner_precision=32), 1e-9, 1),
+    "vpgmres_mixed_convdiff3d_pe08_192": ("convdiff3d_pe08", "vpgmres", "gmres",
+                                          dict(outer_restart=21, outer_maxloop=5000, outer_eps=1e-10,
+                                               inner_maxloop=11, inner_restart=12, inner_eps=0.0,
+                                               inner_precision=32), 1e-9, 1),
 }
 
 _problems = {}
