@@ -12,7 +12,8 @@ generated inputs).
   steps (-R 21 -l 11 -r 12), to 1e-10 (vpgmres.cpp:746-975).
 - 3-D 7-point convection-diffusion 192^3, cell Peclet 0.8 (the adjoint
   configuration's forward solve; its columns reach 36,864 rows, so the SpMV
-  reads int32 columns): the same mixed VPGMRES, to 1e-10.
+  reads int32 columns): the same mixed VPGMRES, to 1e-10; and the backward
+  solve, GMRES(20) on its explicit transpose, to 1e-10.
 
 Bar: the same exit status, the reported iteration count within +-1 (k-skip:
 +-(k+1), since its residual is only checked once per block of k+1 steps),
@@ -139,3 +140,41 @@ def test_bicg_full_size_within_reference_spread(ctx, oracle):
     assert len(h) > m and np.array_equal(h[:m, 0], ho8[:m, 0])
     assert np.allclose(h[:m, 1], ho8[:m, 1], rtol=1e-6, atol=0)
 
+
+
+def test_adjoint_backward_solve_full_size(ctx, oracle):
+    # the adjoint configuration's backward solve: GMRES(20) (-R 21) on the
+    # explicit A^T of the 192^3 Peclet-0.8 matrix, to 1e-10, GPU against the
+    # oracle on the same transposed CSR (the oracle's own counting-sort
+    # transpose; the GPU builds A^T itself and must agree with it bit for bit)
+    import torch
+    from solver_cpp_amd import api
+    P = _problem("convdiff3d_pe08")
+    ptr, col, val = P["ptr"], P["col"], P["val"]
+    tp, tc, tv = oracle.transpose(ptr, col, val)
+    g = np.random.default_rng(57).standard_normal(len(ptr) - 1)
+    kw = dict(outer_eps=1e-10, outer_maxloop=20000, outer_restart=21)
+    A = api.Matrix(ctx, ptr, col, val, transpose=True)
+    gtp, gtc, gtv = A.transpose_csr()
+    assert np.array_equal(gtp, tp) and np.array_equal(gtc, tc) and np.array_equal(gtv, tv)
+    A.close()
+    At = api.Matrix(ctx, tp, tc, tv)
+    x, r, h = api.solve(ctx, At, torch.from_numpy(g).cuda(), "gmres", hist_cap=1 << 16, **kw)
+    x = x.cpu().numpy()
+    At.close()
+    xo, ro, ho = oracle.solve(tp, tc, tv, g, "gmres", hist_cap=1 << 16,
+                              omp_threads=min(oracle.max_threads(), 16), **kw)
+    xdiff = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
+    path = os.environ.get("KS_PARITY_REPORT")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": "gmres20_transpose_convdiff3d_pe08_192", "iters": r["iters"],
+                                "oracle_iters": ro["iters"], "x_rel_diff": xdiff,
+                                "relres_true": r["relres_true"]}) + "\n")
+    print(f"[parity-full] GMRES(20) on A^T 192^3: iters {r['iters']} / oracle {ro['iters']}, x rel diff {xdiff:.3e}",
+          flush=True)
+    assert r["status"] == ro["status"] == 0
+    assert abs(r["iters"] - ro["iters"]) <= 1
+    assert xdiff <= 1e-10
+    m = min(len(h), len(ho))
+    assert m > 0 and np.array_equal(h[:m, 0], ho[:m, 0])
