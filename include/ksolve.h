@@ -80,7 +80,18 @@ typedef enum ks_status {
                                  32767 columns of their row (stencils, banded
                                  matrices) also gets int16 offsets col - row,
                                  which the workgroup-tile SpMV reads instead
-                                 (2 bytes per entry less; same results) */
+                                 (2 bytes per entry less; same results).
+                                 Otherwise, when A's (or A^T's) rows are
+                                 sorted, each row's first column lies in
+                                 [row - 49152, row + 16383], the gaps between
+                                 its columns are below 65536 and no tile
+                                 averages over 8 entries per row, that side
+                                 gets delta-coded 16-bit columns instead
+                                 (3-D stencils up to 221^3; same results) */
+#define KS_MAT_COL_DELTA 4096u /* prefer the delta-coded 16-bit columns over
+                                  int16 offsets where both apply (they read
+                                  the same bytes; int16 offsets have no
+                                  tile condition) */
 #define KS_MAT_BORROW   256u /* with KS_PTR_DEVICE: use the caller's ptr/col/val
                                 in place (no copy); they must outlive the
                                 matrix, col and val must be 16-byte aligned,
@@ -183,8 +194,8 @@ void ks_mat_destroy(ks_mat* m);
 ks_status ks_mat_info(const ks_mat* m, int64_t* n, int64_t* nnz,
                       int32_t* spmv_kind, int32_t* lanes_per_row);
 /* Bytes per column index that the SpMV of A (trans = 0) or A^T (trans = 1)
- * reads, in a precision (KS_F64 / KS_F32): 2 with int16 offsets (see
- * KS_MAT_COL32), else 4.  No reference counterpart (the reference always
+ * reads, in a precision (KS_F64 / KS_F32): 2 with int16 offsets or
+ * delta-coded columns (see KS_MAT_COL32), else 4.  No reference counterpart (the reference always
  * reads int32 columns, cudaFunction.cu:127-156); the bench uses it for the
  * SpMV's byte count. */
 ks_status ks_mat_index_bytes(const ks_mat* m, int32_t trans, int32_t precision, int32_t* bytes);

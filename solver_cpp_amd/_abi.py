@@ -30,6 +30,7 @@ KS_MAT_BORROW = 256
 KS_MAT_WAVE_TILES = 512
 KS_MAT_PANELS = 1024
 KS_MAT_COL32 = 2048
+KS_MAT_COL_DELTA = 4096
 
 KS_F64 = 64
 KS_F32 = 32

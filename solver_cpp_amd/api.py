@@ -182,7 +182,8 @@ class Matrix:
     # column indices kept in the SpMV (without it, banded matrices are read
     # with int16 offsets)
     LAYOUTS = {"auto": 0, "col32": _abi.KS_MAT_COL32, "stream": _abi.KS_MAT_WG_TILES, "wg": _abi.KS_MAT_WG_TILES,
-               "wg_col32": _abi.KS_MAT_WG_TILES | _abi.KS_MAT_COL32,
+               "wg_col32": _abi.KS_MAT_WG_TILES | _abi.KS_MAT_COL32, "delta": _abi.KS_MAT_COL_DELTA,
+               "wg_delta": _abi.KS_MAT_WG_TILES | _abi.KS_MAT_COL_DELTA,
                "wave": _abi.KS_MAT_WAVE_TILES, "vector": _abi.KS_MAT_VECTOR, "merge": _abi.KS_MAT_MERGE,
                "panels": _abi.KS_MAT_PANELS, "panels_wg": _abi.KS_MAT_PANELS | _abi.KS_MAT_WG_TILES,
                "panels_wave": _abi.KS_MAT_PANELS | _abi.KS_MAT_WAVE_TILES,

@@ -165,6 +165,10 @@ struct TileSet {
   int64_t ntiles = 0;
   DBuf<int32_t> tiles;  // tile row starts, ntiles + 1
   DBuf<int32_t> tnz;    // tile entry starts (ptr[tiles[t]]), ntiles + 1
+  // workgroup tiles: k_spmv_stream sums every tile's rows one per lane (no
+  // tile averages over 8 entries per row across more than 128 rows, none is
+  // one long row), the condition of its delta-coded column format
+  bool lane1 = false;
 };
 
 // SpMV plan of one CSR matrix: CSR-stream tiles (kind 0), L lanes per row
@@ -186,10 +190,12 @@ struct MatView {
   int64_t n = 0, nnz = 0;
   const int32_t* ptr = nullptr;
   const int32_t* col = nullptr;
-  // int16 offsets col - row, when the matrix has them (banded matrices) and
-  // this view's SpMV reads them (workgroup CSR-stream tiles); ib = bytes per
+  // 16-bit column indices, when the matrix has them and this view's SpMV
+  // reads them (workgroup CSR-stream tiles): int16 offsets col - row (banded
+  // matrices) or delta-coded columns (see k_spmv_stream); ib = bytes per
   // column index that the SpMV reads
   const int16_t* col16 = nullptr;
+  const uint16_t* cold = nullptr;
   int ib = 4;
   const T* val = nullptr;
   int kind = 0;
@@ -245,6 +251,7 @@ struct ks_mat {
   uint32_t flags = 0;
   ks::DBuf<int32_t> ptr, col;
   ks::DBuf<int16_t> col16, tcol16;  // col - row of A and A^T (banded matrices)
+  ks::DBuf<uint16_t> cold, tcold;   // delta-coded columns of A and A^T
   ks::DBuf<double> val;
   ks::DBuf<float> val32;
   bool has_t = false;
@@ -260,8 +267,11 @@ struct ks_mat {
   // hands the int16 offsets to a view whose SpMV reads them
   template <class T>
   void set_col16(ks::MatView<T>& v, bool trans) const {
+    if (v.kind != 0 || !v.wg_tiles) return;
     const int16_t* p = trans ? tcol16.p : col16.p;
-    if (p && v.kind == 0 && v.wg_tiles) { v.col16 = p; v.ib = 2; }
+    const uint16_t* q = trans ? tcold.p : cold.p;
+    if (p) { v.col16 = p; v.ib = 2; }
+    else if (q) { v.cold = q; v.ib = 2; }
   }
 };
 
@@ -391,22 +401,33 @@ int spmv_L(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, i
   return G;
 }
 
+// The workgroup CSR-stream SpMV in the view's column index format, on G
+// workgroups.  G is always the int32 kernel's grid (grid_rows of that
+// instantiation): the grid, and so the reduction partials and their
+// summation order, is the same in every index format, so a solve's results
+// do not depend on the format (the 16-bit kernels, which need fewer
+// registers, may have room for more workgroups than they are given).
+template <class T, int NX, int NV, class Epi>
+void launch_stream(ks_ctx* c, int cls, const MatView<T>& A, int G, const T* x0, const T* x1, const Epi& epi, Gate g,
+                   double bytes) {
+  if (A.col16)
+    klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi, int16_t>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col16,
+            A.val, x0, x1, epi, g, red(c), stat(c, cls, bytes));
+  else if (A.cold)
+    klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi, uint16_t>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.cold,
+            A.val, x0, x1, epi, g, red(c), stat(c, cls, bytes));
+  else
+    klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
+            epi, g, red(c), stat(c, cls, bytes));
+}
+
 template <class T, int NX, int NV, class Epi>
 int spmv_stream(ks_ctx* c, const MatView<T>& A, const T* x0, const T* x1, const Epi& epi, Gate g, int cls,
                 double bytes) {
   int G;
   if (A.wg_tiles) {
-    // the grid (and so the reduction partials and their summation order) is
-    // that of the int32 kernel in both index formats, so a solve's results do
-    // not depend on the format: the int16 kernel, which needs fewer
-    // registers, may have room for more workgroups than it is given
     G = grid_rows(c, k_spmv_stream<T, NX, NV, Epi>, A.ntiles);
-    if (A.col16)
-      klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi, int16_t>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col16,
-              A.val, x0, x1, epi, g, red(c), stat(c, cls, bytes));
-    else
-      klaunch(c, cls, k_spmv_stream<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0,
-              x1, epi, g, red(c), stat(c, cls, bytes));
+    launch_stream<T, NX, NV, Epi>(c, cls, A, G, x0, x1, epi, g, bytes);
   } else {
     G = grid_rows(c, k_spmv_wave<T, NX, NV, Epi>, (A.ntiles + kBlock / 64 - 1) / (kBlock / 64));
     klaunch(c, cls, k_spmv_wave<T, NX, NV, Epi>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col, A.val, x0, x1,
@@ -560,6 +581,14 @@ inline void upload_tiles(ks_ctx* c, TileSet& ts, const std::vector<int32_t>& t, 
 inline void tile_set(ks_ctx* c, TileSet& ts, const int32_t* hp, int64_t n, int64_t rows, int64_t nnz_max) {
   std::vector<int32_t> t = build_tiles(hp, n, rows, nnz_max), z(t.size());
   for (size_t i = 0; i < t.size(); ++i) z[i] = hp[t[i]];
+  // k_spmv_stream's lanes per row for a tile of nr rows and cnt entries is 1
+  // unless 2 nr <= kBlock and 8 nr < cnt; a tile over kTileNnz entries is
+  // one long row
+  ts.lane1 = true;
+  for (size_t i = 0; i + 1 < t.size() && ts.lane1; ++i) {
+    const int64_t nr = t[i + 1] - t[i], cnt = (int64_t)z[i + 1] - z[i];
+    if (cnt > kTileNnz || (2 * nr <= kBlock && 8 * nr < cnt)) ts.lane1 = false;
+  }
   upload_tiles(c, ts, t, z);
 }
 

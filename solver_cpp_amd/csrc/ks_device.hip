@@ -211,16 +211,44 @@ bool offsets_fit16(int64_t n, const int32_t* hp, const int32_t* hc) {
   return ok;
 }
 
-void build_col16(ks_mat* m) {
+void build_col16(ks_mat* m, DBuf<int16_t>& out, const int32_t* ptr, const int32_t* col) {
   ks_ctx* c = m->ctx;
-  m->col16.alloc(m->nnz);
-  k_col_offsets<<<grid_map(m->n), kBlock, 0, c->stream>>>(m->n, m->ptr.p, m->col.p, m->col16.p);
+  out.alloc(m->nnz);
+  k_col_offsets<<<grid_map(m->n), kBlock, 0, c->stream>>>(m->n, ptr, col, out.p);
   KS_HIPCK(hipGetLastError());
-  if (m->has_t) {
-    m->tcol16.alloc(m->nnz);
-    k_col_offsets<<<grid_map(m->n), kBlock, 0, c->stream>>>(m->n, m->tptr.p, m->tcol.p, m->tcol16.p);
-    KS_HIPCK(hipGetLastError());
-  }
+}
+
+// Delta-coded 16-bit columns (k_spmv_stream's uint16_t format) of the CSR
+// (ptr, col) with plan pl, when its workgroup tiles sum one row per lane and
+// every row encodes (sorted, first column and gaps in range: checked on the
+// device, as A^T's columns exist only there).  Returns false, with `out`
+// empty, otherwise.
+bool build_col_delta(ks_mat* m, const Plan& pl, DBuf<uint16_t>& out, const int32_t* ptr, const int32_t* col) {
+  if (pl.kind != 0 || !(pl.wg64 || pl.wg32) || !pl.wg.lane1) return false;
+  ks_ctx* c = m->ctx;
+  out.alloc(m->nnz);
+  DBuf<int> bad(1);
+  KS_HIPCK(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
+  k_col_delta<<<grid_map(m->n), kBlock, 0, c->stream>>>(m->n, ptr, col, out.p, bad.p);
+  KS_HIPCK(hipGetLastError());
+  int hb = 0;
+  KS_HIPCK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  KS_HIPCK(hipStreamSynchronize(c->stream));
+  if (hb) out.release();
+  return !hb;
+}
+
+// The SpMV's 16-bit column indices of A and A^T: int16 offsets when every
+// offset fits (off16), delta-coded columns where they apply, in the order
+// KS_MAT_COL_DELTA asks for; int32 columns otherwise.
+void build_index16(ks_mat* m, bool off16) {
+  const bool delta_first = (m->flags & KS_MAT_COL_DELTA) != 0;
+  auto side = [&](DBuf<int16_t>& o16, DBuf<uint16_t>& od, const Plan& pl, const int32_t* ptr, const int32_t* col) {
+    if (off16 && !delta_first) build_col16(m, o16, ptr, col);
+    else if (!build_col_delta(m, pl, od, ptr, col) && off16) build_col16(m, o16, ptr, col);
+  };
+  side(m->col16, m->cold, m->plan, m->ptr.p, m->col.p);
+  if (m->has_t) side(m->tcol16, m->tcold, m->tplan, m->tptr.p, m->tcol.p);
 }
 
 // Split the CSR (hp, hc: host row pointers and columns of A or A^T) into
@@ -856,13 +884,15 @@ ks_status ks_mat_create_csr(ks_ctx* c, int64_t n, int64_t nnz, const int32_t* pt
   for (int64_t j = 0; j < nnz; ++j)
     KS_CHECK(hc[j] >= 0 && hc[j] < n, KS_ERR_BAD_INPUT, "column index out of range at entry " + std::to_string(j));
   m->max_row = mx;
+  KS_CHECK(!((flags & KS_MAT_COL32) && (flags & KS_MAT_COL_DELTA)), KS_ERR_BAD_INPUT,
+           "KS_MAT_COL32 and KS_MAT_COL_DELTA exclude each other");
   const bool off16 = !(flags & KS_MAT_COL32) && nnz > 0 && offsets_fit16(n, hp.data(), hc.data());
   make_plan(c, m->plan, hp.data(), n, nnz, (flags & KS_MAT_VECTOR) != 0, tile_choice(flags),
               merge_choice(flags, n, nnz, mx));
   if (nnz > 0 && panel_choice(flags, n, hp.data(), hc.data()))
     build_panels(m.get(), m->panels, hp.data(), hc.data(), nullptr, flags);
   if (flags & KS_MAT_TRANSPOSE) build_transpose(m.get());
-  if (off16) build_col16(m.get());
+  if (!(flags & KS_MAT_COL32) && nnz > 0) build_index16(m.get(), off16);
   if (flags & KS_MAT_FP32_COPY) ensure_fp32(m.get());
   KS_HIPCK(hipStreamSynchronize(c->stream));
   *out = m.release();

@@ -444,8 +444,16 @@ struct HasProj<E, std::void_t<decltype(E::kProj)>> : std::true_type {};
 
 // IX = int16_t: the column index arrives as the offset col - row (every
 // offset of the matrix in [-32767, 32767]; stencil and other banded
-// matrices), half the index bytes of int32 columns.  The row sums are the
-// same: entry for entry, the same x is gathered in the same order.
+// matrices), half the index bytes of int32 columns.
+// IX = uint16_t: delta-coded columns (matrices whose offsets do not fit
+// int16 but whose rows are sorted with gaps below 65536, such as 3-D stencils
+// with planes over 32767 points): a row's first entry holds col - row +
+// kDeltaBias and every later one the gap to the previous column, decoded by
+// the one lane that sums the row (the host builds this format only when
+// every tile sums its rows one per lane and no row is a tile of its own).
+// In every format the row sums are the same: entry for entry, the same x is
+// gathered in the same order.
+constexpr int32_t kDeltaBias = 49152;
 template <class T, int NX, int NV, class Epi, class IX = int32_t>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves<T, NX, NV>))) void k_spmv_stream(int64_t n, int64_t ntiles, const int32_t* __restrict__ tiles,
                                                         const int32_t* __restrict__ tnz,
@@ -456,14 +464,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
                                                         StatArgs st) {
   if (g.off()) return;
   st.count();
-  constexpr bool kRel = sizeof(IX) == 2;
+  constexpr bool kRel = std::is_same<IX, int16_t>::value;
+  constexpr bool kDelta = std::is_same<IX, uint16_t>::value;
   // With the projection epilogue, the y of the workgroup's first kYT tiles
   // is kept in LDS (tile s at sy[s][row - (r0 & ~3)], 16-byte aligned row
   // chunks): the projection reads it there instead of from L2/HBM, and it
   // goes to HBM in one burst per tile during the projection rather than
   // between the matrix reads.  kYT is what fits beside the tile buffers at
   // the kernel's occupancy (7 FP32 / 5 FP64 workgroups per CU in 160 KB).
-  constexpr int kYT = !HasProj<Epi>::value ? 0 : sizeof(T) == 4 ? (kRel ? 9 : 4) : (kRel ? 4 : 2);
+  constexpr int kYT = !HasProj<Epi>::value ? 0 : sizeof(T) == 4 ? (sizeof(IX) == 2 ? 9 : 4) : (sizeof(IX) == 2 ? 4 : 2);
   constexpr int kYW = kTileRows + 4;
   __shared__ __align__(16) T sv[kTileWin];
   __shared__ __align__(16) IX sc[kTileWin];
@@ -504,7 +513,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
       T s0 = 0, s1 = 0;
       for (int k = tid; k < cnt; k += kBlock) {
         const T a = ld_stream(val + j0 + k);
-        const int c = (int)ld_stream(col + j0 + k) + (kRel ? r0 : 0);
+        int c;
+        if constexpr (kDelta) {
+          // not reached (no row is a tile of its own in this format); kept
+          // correct: the column is the prefix sum of the row's codes
+          c = r0 - kDeltaBias;
+          for (int q = 0; q <= k; ++q) c += (int)col[j0 + q];
+        } else {
+          c = (int)ld_stream(col + j0 + k) + (kRel ? r0 : 0);
+        }
         s0 += a * x0[c];
         if (NX == 2) s1 += a * x1[c];
       }
@@ -544,17 +561,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWgWaves
       if (more) d.load(ptr, col, val, nr0, nr1, nj0, nj1, nnz, tid);
       if (more2) { mr0 = tiles[tnn]; mr1 = tiles[tnn + 1]; mj0 = tnz[tnn]; mj1 = tnz[tnn + 1]; }
       // more than one thread per row only for rows averaging > 8 entries
+      // (delta-coded columns: always one per row, which the host checked is
+      // the choice below for every tile)
       int tpr = 1;
-      while (tpr < 64 && tpr * 2 * nr <= kBlock && tpr * 8 * nr < cnt) tpr *= 2;
+      if constexpr (!kDelta)
+        while (tpr < 64 && tpr * 2 * nr <= kBlock && tpr * 8 * nr < cnt) tpr *= 2;
       const int grp = tid / tpr, lane = tid % tpr;
       if (grp < nr) {
         const int s = sptr[grp], e = sptr[grp + 1];
-        const int cb = kRel ? r0 + grp : 0;
+        const int cb = kRel ? r0 + grp : kDelta ? r0 + grp - kDeltaBias : 0;
+        int cd = cb;  // delta-coded: the running column
         T a0 = 0, a1 = 0;
 #pragma unroll 2
         for (int k = s + lane; k < e; k += tpr) {
           const T v = sv[k];
-          const int c = cb + (int)sc[k];
+          int c;
+          if constexpr (kDelta) {
+            cd += (int)sc[k];
+            c = cd;
+          } else {
+            c = cb + (int)sc[k];
+          }
           a0 = mul_add_rn(a0, v, x0[c]);
           if (NX == 2) a1 = mul_add_rn(a1, v, x1[c]);
         }
@@ -1693,6 +1720,23 @@ __global__ void k_col_offsets(int64_t n, const int32_t* __restrict__ ptr, const 
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride)
     for (int32_t j = ptr[r], e = ptr[r + 1]; j < e; ++j) out[j] = (int16_t)(col[j] - r);
+}
+
+// Delta-coded columns (see k_spmv_stream): out[first] = col - r + kDeltaBias,
+// out[j] = col[j] - col[j - 1] after it; *bad is set when a row is not
+// sorted or a code does not fit 16 bits (the host then keeps int32 columns)
+__global__ void k_col_delta(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            uint16_t* __restrict__ out, int* __restrict__ bad) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    int64_t prev = r - kDeltaBias;
+    for (int32_t j = ptr[r], e = ptr[r + 1]; j < e; ++j) {
+      const int64_t d = (int64_t)col[j] - prev;
+      if (d < 0 || d > 65535) *bad = 1;
+      out[j] = (uint16_t)d;
+      prev = col[j];
+    }
+  }
 }
 
 // out[0..m) = v: clears memory with a kernel (no memset node), for state

@@ -796,14 +796,9 @@ void launch_spmv_proj(ks_ctx* c, const MatView<T>& A, const T* vk, T* w, const d
     if (pn.pk >= 0) {
       e.np = c->part2.p; e.ng = pn.ng;
     }
-    // the int32 kernel's grid in both index formats (see spmv_stream)
+    // the int32 kernel's grid in every index format (see launch_stream)
     const int G = grid_rows(c, k_spmv_stream<T, 1, 0, EpiDivProj<T>>, A.ntiles);
-    if (A.col16)
-      klaunch(c, cls, k_spmv_stream<T, 1, 0, EpiDivProj<T>, int16_t>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr,
-              A.col16, A.val, vk, vk, e, g, red(c), stat(c, cls, bytes));
-    else
-      klaunch(c, cls, k_spmv_stream<T, 1, 0, EpiDivProj<T>>, G, 0, A.n, A.ntiles, A.tiles, A.tnz, A.ptr, A.col,
-              A.val, vk, vk, e, g, red(c), stat(c, cls, bytes));
+    launch_stream<T, 1, 0, EpiDivProj<T>>(c, cls, A, G, vk, vk, e, g, bytes);
     if (pn.pk >= 0) launch_proj_finish(c, G, nv, FinNormProj{FinProj{hcol, ldh, nv, vn}, pn.c, pn.pk,
                                                              c->part2.p, pn.ng, 0.0}, g);
     else launch_proj_finish(c, G, nv, FinProj{hcol, ldh, nv, vn}, g);

@@ -21,7 +21,7 @@ def _mat(ctx, ptr, col, val, **kw):
     return api.Matrix(ctx, ptr, col, val, **kw)
 
 
-LAYOUTS = ["auto", "col32", "stream", "wg", "wg_col32", "wave", "vector", "merge", "panels"]
+LAYOUTS = ["auto", "col32", "delta", "stream", "wg", "wg_col32", "wg_delta", "wave", "vector", "merge", "panels"]
 
 
 def _auto_kind(ptr):
@@ -42,8 +42,8 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     n = len(ptr) - 1
     x = np.random.default_rng(7).standard_normal(n)
     A = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout=layout)
-    kinds = {"auto": _auto_kind(ptr), "col32": _auto_kind(ptr), "stream": 0, "wg": 0, "wg_col32": 0, "wave": 0,
-             "vector": 1, "merge": 2, "panels": 3}
+    kinds = {"auto": _auto_kind(ptr), "col32": _auto_kind(ptr), "delta": _auto_kind(ptr), "stream": 0, "wg": 0,
+             "wg_col32": 0, "wg_delta": 0, "wave": 0, "vector": 1, "merge": 2, "panels": 3}
     assert A.info()["spmv_kind"] == kinds[layout]
     m = np.diff(ptr).max()
     y = A.spmv(x)
@@ -69,7 +69,7 @@ def test_spmv_fp64_fp32_and_transpose(ctx, oracle, case, layout):
     assert np.all(np.abs(y32.astype(np.float64) - yo32.astype(np.float64)) <= b32)
 
 
-@pytest.mark.parametrize("layout", ["stream", "wg", "wg_col32", "wave"])
+@pytest.mark.parametrize("layout", ["stream", "wg", "wg_col32", "wg_delta", "wave"])
 @pytest.mark.parametrize("case", ["poisson2d", "convdiff3d", "aniso"])
 def test_spmv_bitexact_short_rows(ctx, oracle, case, layout):
     # rows of <= 8 entries are summed sequentially in CSR order from products
@@ -101,14 +101,14 @@ def _band_csr(n, offsets, seed):
 @pytest.mark.parametrize("reach", [32767, 32768])
 def test_spmv_col16_offset_limit(ctx, oracle, reach):
     # entries at row -+ reach: within int16 offsets at 32767 (the SpMV then
-    # reads col - row as int16, A^T too), int32 columns at 32768; both
-    # bit-exact against the oracle (rows of <= 8 entries) and equal to the
-    # int32 path
+    # reads col - row as int16, A^T too), delta-coded columns at 32768 (and
+    # int32 columns when those are not allowed either); all bit-exact
+    # against the oracle (rows of <= 8 entries) and equal to the int32 path
     n = 70_000
     ptr, col, val = _band_csr(n, [-reach, -3, 0, 1, reach], 21)
     A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
     B = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout="col32")
-    ib = 2 if reach <= 32767 else 4
+    ib = 2
     for prec in (64, 32):
         assert A.index_bytes(False, prec) == ib and A.index_bytes(True, prec) == ib
         assert B.index_bytes(False, prec) == 4 and B.index_bytes(True, prec) == 4
@@ -131,11 +131,93 @@ def test_spmv_col16_long_rows(ctx, oracle):
     assert np.diff(ptr).max() > 2048
     A = _mat(ctx, ptr, col, val, fp32=True, layout="wg")
     B = _mat(ctx, ptr, col, val, fp32=True, layout="wg_col32")
-    assert A.index_bytes() == 2 and B.index_bytes() == 4
+    # delta-coded columns do not apply (tiles of one long row): int16 offsets
+    C = _mat(ctx, ptr, col, val, fp32=True, layout="wg_delta")
+    assert A.index_bytes() == 2 and B.index_bytes() == 4 and C.index_bytes() == 2
     x = np.random.default_rng(23).standard_normal(len(ptr) - 1)
-    assert np.array_equal(A.spmv(x), B.spmv(x))
+    assert np.array_equal(A.spmv(x), B.spmv(x)) and np.array_equal(C.spmv(x), B.spmv(x))
     x32 = x.astype(np.float32)
     assert np.array_equal(A.spmv(x32, precision=32), B.spmv(x32, precision=32))
+    assert np.array_equal(C.spmv(x32, precision=32), B.spmv(x32, precision=32))
+
+
+def _check_same_as_int32(ctx, oracle, ptr, col, val, A, seed):
+    # A's SpMV (A and A^T, FP64 and FP32) bit-exact against the oracle (rows
+    # of <= 8 entries: the oracle's order) and equal to the int32 columns'
+    B = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout="col32")
+    n = len(ptr) - 1
+    x = np.random.default_rng(seed).standard_normal(n)
+    y = A.spmv(x)
+    assert np.array_equal(y, oracle.spmv(ptr, col, val, x)) and np.array_equal(y, B.spmv(x))
+    otp, otc, otv = oracle.transpose(ptr, col, val)
+    yt = A.spmv(x, trans=True)
+    assert np.array_equal(yt, oracle.spmv(otp, otc, otv, x)) and np.array_equal(yt, B.spmv(x, trans=True))
+    x32 = x.astype(np.float32)
+    y32 = A.spmv(x32, precision=32)
+    assert np.array_equal(y32, oracle.spmv_f32(ptr, col, val.astype(np.float32), x32))
+    assert np.array_equal(y32, B.spmv(x32, precision=32))
+    yt32 = A.spmv(x32, precision=32, trans=True)
+    assert np.array_equal(yt32, oracle.spmv_f32(otp, otc, otv.astype(np.float32), x32))
+    assert np.array_equal(yt32, B.spmv(x32, precision=32, trans=True))
+    B.close()
+
+
+@pytest.mark.parametrize("offsets,ib,ibt", [
+    ([-49152, -3, 0, 1, 49152], 2, 2),   # first column at row - 49152: the lowest code
+    ([-49153, -3, 0, 1, 49153], 4, 4),   # one below
+    ([0, 65535], 2, 4),                  # gap 65535 in A; A^T's first column at row - 65535
+    ([0, 65536], 4, 4),                  # gap 65536
+    ([16383, 40000], 2, 2),              # first column at row + 16383: the highest code
+    ([16384, 40000], 4, 2),              # one above in A; A^T's columns at row - 40000, row - 16384
+])
+def test_spmv_delta_columns_limits(ctx, oracle, offsets, ib, ibt):
+    # offsets beyond int16: the SpMV reads delta-coded 16-bit columns where
+    # every row encodes (the first column within [row - 49152, row + 16383],
+    # sorted, gaps below 65536), A and A^T separately, else int32 columns;
+    # the same sums either way
+    n = 140_000
+    ptr, col, val = _band_csr(n, offsets, 24)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    for prec in (64, 32):
+        assert A.index_bytes(False, prec) == ib and A.index_bytes(True, prec) == ibt
+    _check_same_as_int32(ctx, oracle, ptr, col, val, A, 25)
+    A.close()
+
+
+def test_spmv_delta_columns_unsorted_rows(ctx, oracle):
+    # rows of A in descending column order (a negative gap) keep int32
+    # columns; A^T, built sorted, still gets delta-coded columns
+    n = 100_000
+    ptr, col, val = _band_csr(n, [-40000, -1, 0, 1, 40000], 26)
+    for i in range(n):
+        col[ptr[i]:ptr[i + 1]] = col[ptr[i]:ptr[i + 1]][::-1].copy()
+        val[ptr[i]:ptr[i + 1]] = val[ptr[i]:ptr[i + 1]][::-1].copy()
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    assert A.index_bytes(False, 64) == 4 and A.index_bytes(True, 64) == 2
+    _check_same_as_int32(ctx, oracle, ptr, col, val, A, 27)
+    A.close()
+
+
+def test_spmv_delta_columns_tile_condition(ctx, oracle):
+    # 18 entries per row: the SpMV sums a tile's rows with two lanes each,
+    # where delta-coded columns (one lane per row) do not apply, so offsets
+    # beyond int16 keep int32 columns, and offsets within int16 keep int16
+    # offsets even when delta-coded columns are preferred
+    n = 100_000
+    wide = [-40000] + list(range(-8, 8)) + [40000]
+    ptr, col, val = _band_csr(n, wide, 28)
+    A = _mat(ctx, ptr, col, val, transpose=True, fp32=True)
+    assert A.index_bytes(False, 64) == 4 and A.index_bytes(True, 64) == 4
+    x = np.random.default_rng(29).standard_normal(n)
+    B = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout="col32")
+    assert np.array_equal(A.spmv(x), B.spmv(x)) and np.array_equal(A.spmv(x, trans=True), B.spmv(x, trans=True))
+    ptr, col, val = _band_csr(n, [-30000] + list(range(-8, 8)) + [30000], 30)
+    C = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout="delta")
+    D = _mat(ctx, ptr, col, val, transpose=True, fp32=True, layout="col32")
+    assert C.index_bytes(False, 64) == 2 and C.index_bytes(True, 64) == 2
+    assert np.array_equal(C.spmv(x), D.spmv(x)) and np.array_equal(C.spmv(x, trans=True), D.spmv(x, trans=True))
+    for M in (A, B, C, D):
+        M.close()
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)

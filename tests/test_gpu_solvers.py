@@ -349,22 +349,27 @@ def test_cli_solver_end_to_end(tmp_path, oracle):
                                          inner_restart=12, inner_eps=0.0, inner_precision=32)),
     ("aniso", "cg", "NO", dict(outer_eps=1e-10, outer_maxloop=20000)),
 ])
-def test_solve_same_in_both_index_formats(ctx, mat, outer, inner, kw):
+def test_solve_same_in_every_index_format(ctx, mat, outer, inner, kw):
     # a matrix large enough that the SpMV grids fill the GPU (more tiles than
-    # resident workgroups): with int16 column offsets and with int32 columns
-    # the solve is the same, bit for bit - same grids, same partial sums
+    # resident workgroups): with int16 column offsets, delta-coded 16-bit
+    # columns and int32 columns the solve is the same, bit for bit - same
+    # grids, same partial sums
     from solver_cpp_amd import api
     ptr, col, val = P.aniso(740)[:3] if mat == "aniso" else P.poisson2d(740)[:3]
     n = len(ptr) - 1
     assert n // 256 > 2048
     b = np.random.default_rng(31).uniform(-1, 1, n)
     A = api.Matrix(ctx, ptr, col, val, fp32=True, spd=True)
+    C = api.Matrix(ctx, ptr, col, val, fp32=True, spd=True, layout="delta")
     B = api.Matrix(ctx, ptr, col, val, fp32=True, spd=True, layout="col32")
-    assert A.index_bytes(False, 32) == 2 and B.index_bytes(False, 32) == 4
+    assert A.index_bytes(False, 32) == 2 and C.index_bytes(False, 32) == 2 and B.index_bytes(False, 32) == 4
     x, r, h = api.solve(ctx, A, b, outer, inner, **kw)
+    z, t, f = api.solve(ctx, C, b, outer, inner, **kw)
     y, s, g = api.solve(ctx, B, b, outer, inner, **kw)
-    print(f"[formats] {outer}/{inner}: iters {r['iters']} / {s['iters']}", flush=True)
-    assert r["status"] == s["status"] == 0
+    print(f"[formats] {outer}/{inner}: iters {r['iters']} / {t['iters']} / {s['iters']}", flush=True)
+    assert r["status"] == s["status"] == t["status"] == 0
     assert r["iters"] == s["iters"] and np.array_equal(x, y) and np.array_equal(h, g)
+    assert t["iters"] == s["iters"] and np.array_equal(z, y) and np.array_equal(f, g)
     A.close()
     B.close()
+    C.close()
