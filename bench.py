@@ -173,7 +173,7 @@ def main():
                        "bytes_per_launch": bytes_per32, "ms_per_launch": ms_per, "launches": launches,
                        "byte_model": "int32 CSR (8*nnz + 12*n + 4 + 4*n*nv)",
                        "index_bytes": ib,
-                       "format": {"what": "bytes of the format the kernel reads (int16 column offsets when "
+                       "format": {"what": "bytes of the format the kernel reads (16-bit column indices when "
                                           "index_bytes is 2): its use of HBM bandwidth",
                                   "bytes_per_launch": bytes_per, "GBps": round(moved, 1) if moved else None,
                                   "frac": round(moved / HBM_PEAK_GBS, 4) if moved else None},
@@ -203,6 +203,7 @@ def main():
     out["spmv"] = {"poisson3d_160": spmv_line(A, n, nnz, trans=False)}
     A.close()
     if not a.headline_only:
+        out["hbm_ceiling"] = hbm_ceiling(ctx, achieved, moved)
         out["spmv"].update(spmv_configs(ctx))
         out["adjoint"] = adjoint_times(ctx)
     if not a.no_cpu:
@@ -369,6 +370,20 @@ def calib_copy(a):
     dst.copy_(src)
     torch.cuda.synchronize()
     return 2 * src.numel() * 8
+
+
+def hbm_ceiling(ctx, achieved, moved):
+    """What plain streaming kernels reach on this box (1 GiB, best of four
+    grids): the practical ceiling beside the published 8 TB/s, and the
+    roofline kernel's rates as fractions of the read ceiling."""
+    rd = ctx.stream_rate("read")
+    out = {"what": "plain non-temporal 16-byte streaming kernels over 1 GiB (ks_bench_stream), best of 4 grids",
+           "read_GBps": round(rd, 1), "copy_GBps": round(ctx.stream_rate("copy"), 1),
+           "read_frac_of_peak": round(rd / HBM_PEAK_GBS, 4)}
+    if achieved and moved:
+        out["roofline_achieved_frac_of_read"] = round(achieved / rd, 4)
+        out["roofline_format_frac_of_read"] = round(moved / rd, 4)
+    return out
 
 
 def spmv_gbps(A, n, nnz, prec, trans=False, reps=30):

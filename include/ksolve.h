@@ -210,6 +210,13 @@ ks_status ks_spmv(ks_mat* m, int32_t trans, int32_t precision,
  * stream; returns the mean launch duration in ms (x, y device pointers). */
 ks_status ks_spmv_bench(ks_mat* m, int32_t trans, int32_t precision,
                         const void* x, void* y, int32_t reps, double* ms_per_launch);
+/* Measurement only, no reference counterpart: the HBM rate (GB/s) that a
+ * plain streaming kernel reaches on this device over `bytes` (a multiple of
+ * 16, 16 KiB to 16 GiB, allocated and freed inside the call): kind 0 reads,
+ * 1 copies (2 x bytes moved); the best of four grid sizes, each
+ * timed over `reps` launches with hipEvents on the context stream.  The
+ * bench reports it beside the SpMV's fraction of the published peak. */
+ks_status ks_bench_stream(ks_ctx* ctx, int32_t kind, int64_t bytes, int32_t reps, double* gbps);
 /* dot / norm: replaces blas::dot and blas::norm_2 (blas.cpp:31-41,149-209). */
 ks_status ks_dot(ks_ctx* ctx, int64_t n, int32_t precision, const void* x,
                  const void* y, double* out, uint32_t flags);

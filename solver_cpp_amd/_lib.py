@@ -36,6 +36,7 @@ _SIGS = {
     "ks_mat_index_bytes": (_i32, [_vp, _i32, _i32, C.POINTER(_i32)]),
     "ks_spmv": (_i32, [_vp, _i32, _i32, _vp, _vp, _u32]),
     "ks_spmv_bench": (_i32, [_vp, _i32, _i32, _vp, _vp, _i32, C.POINTER(_dbl)]),
+    "ks_bench_stream": (_i32, [_vp, _i32, _i64, _i32, C.POINTER(_dbl)]),
     "ks_dot": (_i32, [_vp, _i64, _i32, _vp, _vp, C.POINTER(_dbl), _u32]),
     "ks_norm2": (_i32, [_vp, _i64, _i32, _vp, C.POINTER(_dbl), _u32]),
     "ks_axpy": (_i32, [_vp, _i64, _dbl, _vp, _vp, _vp, _u32]),

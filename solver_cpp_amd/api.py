@@ -102,6 +102,16 @@ class Context:
     def synchronize(self):
         self.check(self.lib.ks_ctx_synchronize(self.h), "synchronize")
 
+    STREAM_KINDS = {"read": 0, "copy": 1}
+
+    def stream_rate(self, kind="read", nbytes=1 << 30, reps=10):
+        """GB/s that a plain streaming kernel (read or copy of `nbytes`)
+        reaches on this device (ks_bench_stream; measurement only)."""
+        out = C.c_double()
+        self.check(self.lib.ks_bench_stream(self.h, self.STREAM_KINDS[kind], int(nbytes), int(reps), C.byref(out)),
+                   "bench_stream")
+        return out.value
+
     def close(self):
         if getattr(self, "h", None):
             for m in list(self._mats):

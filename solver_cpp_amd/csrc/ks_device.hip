@@ -1025,6 +1025,45 @@ ks_status ks_spmv_bench(ks_mat* m, int32_t trans, int32_t precision, const void*
   KS_API_CATCH(c)
 }
 
+ks_status ks_bench_stream(ks_ctx* c, int32_t kind, int64_t bytes, int32_t reps, double* gbps) {
+  if (!c || !gbps || kind < 0 || kind > 1 || reps <= 0 || bytes < 16 * 1024 || bytes % 16 != 0 ||
+      bytes > (int64_t)16 << 30)
+    return KS_ERR_BAD_INPUT;
+  KS_API_TRY
+  set_device(c);
+  const int64_t n16 = bytes / 16;
+  static constexpr int kGrids[] = {1024, 2048, 4096, 8192};
+  DBuf<StreamChunk> x(n16), y(kind == 1 ? n16 : 0);
+  DBuf<float> part((size_t)kGrids[3] * (kBlock / 64));
+  k_stream_fill<<<grid_map(n16), kBlock, 0, c->stream>>>(n16, x.p, 1.0f);
+  KS_HIPCK(hipGetLastError());
+  hipEvent_t a, b;
+  KS_HIPCK(hipEventCreate(&a));
+  KS_HIPCK(hipEventCreate(&b));
+  double best = 0.0;
+  for (const int G : kGrids) {
+    auto go = [&] {
+      if (kind == 0) k_stream_read<<<G, kBlock, 0, c->stream>>>(n16, x.p, part.p);
+      else k_stream_copy<<<G, kBlock, 0, c->stream>>>(n16, x.p, y.p);
+    };
+    for (int r = 0; r < 2; ++r) go();
+    KS_HIPCK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; ++r) go();
+    KS_HIPCK(hipEventRecord(b, c->stream));
+    KS_HIPCK(hipEventSynchronize(b));
+    KS_HIPCK(hipGetLastError());
+    float t = 0;
+    KS_HIPCK(hipEventElapsedTime(&t, a, b));
+    const double moved = (double)bytes * (kind == 1 ? 2 : 1) * reps;
+    if (t > 0) best = std::max(best, moved / (t * 1e-3) / 1e9);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *gbps = best;
+  return KS_OK;
+  KS_API_CATCH(c)
+}
+
 ks_status ks_dot(ks_ctx* c, int64_t n, int32_t precision, const void* x, const void* y, double* out,
                  uint32_t flags) {
   if (!c || !out || n < 0) return KS_ERR_BAD_INPUT;

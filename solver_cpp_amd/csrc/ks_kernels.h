@@ -1747,6 +1747,35 @@ __global__ void k_fill(int64_t m, T* __restrict__ out, T v) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) out[j] = v;
 }
 
+// Bandwidth references for ks_bench_stream: a plain grid-stride read of
+// 16-byte chunks (one partial per wavefront, so the loads stay live) and a
+// copy, non-temporal, with nothing else in the loop; the fill initialises
+// the source.
+using StreamChunk = Vec16<float>::type;
+__global__ __launch_bounds__(kBlock) void k_stream_read(int64_t n16, const StreamChunk* __restrict__ x,
+                                                        float* __restrict__ out) {
+  float s = 0.f;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) {
+    const StreamChunk v = __builtin_nontemporal_load(x + i);
+    s += (v[0] + v[1]) + (v[2] + v[3]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) out[blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)] = s;
+}
+__global__ __launch_bounds__(kBlock) void k_stream_copy(int64_t n16, const StreamChunk* __restrict__ x,
+                                                        StreamChunk* __restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(x + i), y + i);
+}
+__global__ __launch_bounds__(kBlock) void k_stream_fill(int64_t n16, StreamChunk* __restrict__ y, float v) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const StreamChunk c = {v, v, v, v};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) y[i] = c;
+}
+
 template <class Tin, class Tout>
 __global__ void k_convert(int64_t m, const Tin* __restrict__ in, Tout* __restrict__ out, Gate g) {
   if (g.off()) return;

@@ -438,3 +438,13 @@ def test_sddmm(ctx, oracle):
     g = A.sddmm(-1.0, u, v)
     o = oracle.sddmm(ptr, col, -1.0, u, v)
     assert np.array_equal(g, o)
+
+
+def test_bench_stream(ctx):
+    # the streaming reference kernels run and report a rate; sizes that are
+    # not whole 16-byte chunks are refused
+    from solver_cpp_amd import api
+    for kind in ("read", "copy"):
+        assert ctx.stream_rate(kind, nbytes=64 << 20, reps=2) > 0
+    with pytest.raises(api.KsolveError):
+        ctx.stream_rate("read", nbytes=(64 << 20) + 8)
