@@ -378,6 +378,14 @@ void launch_map(ks_ctx* c, int64_t n, const Op& op, Gate g, int cls, double byte
   klaunch(c, cls, k_map<NV, Op>, G, 0, n, op, g, red(c), stat(c, cls, bytes));
   if constexpr (NV > 0) launch_finish<NV>(c, G, op, g);
 }
+// k_map with its partials written to ra and no finisher: a later kernel
+// forms the totals (partial_totals).  Returns the number of partials.
+template <int NV, class Op>
+int launch_map_partials(ks_ctx* c, int64_t n, const Op& op, Gate g, int cls, double bytes, RedArgs ra) {
+  const int G = grid_map(c, k_map<NV, Op>, n);
+  klaunch(c, cls, k_map<NV, Op>, G, 0, n, op, g, ra, stat(c, cls, bytes));
+  return G;
+}
 
 template <class Op>
 void launch_scalar(ks_ctx* c, const Op& op, Gate g) {
@@ -493,8 +501,11 @@ struct EpiPanelLast {
 };
 
 // y-side epilogue of SpMV plus NV reduction values (finished by k_finish).
+// With finish false no finisher is launched (a later kernel forms the
+// totals of the G partials, G returned).
 template <class T, int NV, class Epi>
-void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, double extra_bytes = 0) {
+int launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gate g, double extra_bytes = 0,
+                bool finish = true) {
   const int cls = sizeof(T) == 8 ? KS_K_SPMV64 : KS_K_SPMV32;
   const double bytes = spmv_bytes<T>(A.n, A.nnz, A.ib) + extra_bytes;
   int G;
@@ -506,7 +517,9 @@ void launch_spmv(ks_ctx* c, const MatView<T>& A, const T* x, const Epi& epi, Gat
   } else {
     G = spmv_dispatch<T, NV>(c, A, x, epi, g, cls, bytes);
   }
-  if constexpr (NV > 0) launch_finish<NV>(c, G, epi, g);
+  if constexpr (NV > 0)
+    if (finish) launch_finish<NV>(c, G, epi, g);
+  return G;
 }
 
 template <class T>

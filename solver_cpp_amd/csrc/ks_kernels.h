@@ -49,6 +49,7 @@ struct Ctl {
   long long tsteps;  // steps over all solves of this level (never reset)
   double eps, bnorm, err;
   double rr, alpha, beta, gamma, dotv, vscale, rz;
+  double rrs[2];   // CG: rr by iteration parity (kernels with a folded finisher)
   double* hist;    // (idx, err) pairs, device
   int* host_stop;  // host-mapped flag mirror of `stop` (nullable)
   int R, ld;       // GMRES restart and H row stride
@@ -145,12 +146,11 @@ struct HasPre : std::false_type {};
 template <class F>
 struct HasPre<F, std::void_t<decltype(std::declval<F&>().pre())>> : std::true_type {};
 
-// F may have pre(), run by every thread before the totals (a workgroup-wide
-// step of its own, such as norm_from_partials).
-template <int NV, class F>
-__global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restrict__ part, F f, Gate g) {
-  if (g.off()) return;
-  if constexpr (HasPre<F>::value) f.pre();
+// The totals themselves, by the whole workgroup (every thread must call it;
+// all get the same LDS array back): k_finish's step, also taken by kernels
+// that fold a finisher into their own start, with the same bits.
+template <int NV>
+__device__ __forceinline__ const double* partial_totals(int G, const double* __restrict__ part) {
   constexpr int NVP = NV <= 4 ? 4 : NV <= 8 ? 8 : NV <= 16 ? 16 : NV <= 32 ? 32 : 64;
   constexpr int TPV = kBlock / NVP;  // 4..64, a divisor of the wavefront
   constexpr int U = 16;
@@ -189,6 +189,16 @@ __global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restri
   for (int off = TPV / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (j == 0) sums[v] = s;
   __syncthreads();
+  return sums;
+}
+
+// F may have pre(), run by every thread before the totals (a workgroup-wide
+// step of its own, such as norm_from_partials).
+template <int NV, class F>
+__global__ __launch_bounds__(kBlock) void k_finish(int G, const double* __restrict__ part, F f, Gate g) {
+  if (g.off()) return;
+  if constexpr (HasPre<F>::value) f.pre();
+  const double* sums = partial_totals<NV>(G, part);
   if (threadIdx.x == 0) f.fin(sums);
 }
 

@@ -75,6 +75,7 @@ struct OpCgInit {
     ctl_reset(c);
     if (set_bnorm) c->bnorm = sqrt(s[0]);
     c->rr = s[0];
+    c->rrs[0] = s[0];
     if (MODE == 0) { c->count = 1; cg_check(c, s[0]); }
     else if (MODE == 1) { c->count = 0; kskip_check(c, s[0]); }
   }
@@ -85,8 +86,8 @@ struct FinAlpha {
   __device__ void operator()(const double* s) const { c->dotv = s[0]; c->alpha = c->rr / s[0]; }
 };
 
-// x += a p; r -= a q; acc (r,r).  MODE 0: CG (beta, rr, count++ and test),
-// 1: VPCG (count++ and test only).
+// x += a p; r -= a q; acc (r,r); the finisher counts the step and tests
+// (VPCG, whose beta comes from its own pass).
 template <class T, int MODE>
 struct OpCgUpdate {
   T *x, *r;
@@ -101,12 +102,70 @@ struct OpCgUpdate {
     acc[0] += (double)ri * (double)ri;
   }
   __device__ void fin(const double* s) {
-    if (MODE == 0) { c->beta = s[0] / c->rr; c->rr = s[0]; }
+    static_assert(MODE == 1, "CG takes OpCgUpdateF / OpXpbyF");
     c->count++;
     c->steps++;
     c->tsteps++;
     cg_check(c, s[0]);
   }
+};
+
+// CG's step (cg.cpp:199-240) with its two one-workgroup finishers folded
+// into the kernels that follow them.  OpCgUpdateF forms alpha = rr / (p, q)
+// from the SpMV's partials (FinAlpha's step) and OpXpbyF forms beta =
+// rr' / rr, the new rr and the stop test from the update's partials, each in
+// every workgroup with partial_totals, the finisher's own summation: the same
+// bits as finisher kernels would give.  Workgroup 0 alone writes the control block.
+// rr is kept by iteration parity: in iteration k both kernels read rrs[k & 1]
+// and OpXpbyF's workgroup 0 writes rrs[(k + 1) & 1], so no workgroup reads a
+// slot that this kernel writes.  The update's partials go to a second buffer
+// (part2), as the SpMV's are still being read while the update's
+// workgroups finish.  Once OpXpbyF has set the stop flag its remaining
+// workgroups may skip p, which the stopped solve no longer reads.
+template <class T>
+struct OpCgUpdateF {
+  T *x, *r;
+  const T *p, *q;
+  Ctl* c;
+  const double* dpart;  // the SpMV's (p, q) partials
+  int dg, slot;
+  double a;
+  __device__ void prep() {
+    const double dot = partial_totals<1>(dg, dpart)[0];
+    a = c->rrs[slot] / dot;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { c->dotv = dot; c->alpha = a; }
+  }
+  __device__ void elem(int64_t i, double* acc) {
+    x[i] = (T)(a * (double)p[i] + (double)x[i]);
+    const T ri = (T)((-a) * (double)q[i] + (double)r[i]);
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+  }
+  __device__ void fin(const double*) {}
+};
+template <class T>
+struct OpXpbyF {
+  T* p;
+  const T* r;
+  Ctl* c;
+  const double* rpart;  // the update's (r, r) partials
+  int rg, slot;
+  double b;
+  __device__ void prep() {
+    const double rr = partial_totals<1>(rg, rpart)[0];
+    b = rr / c->rrs[slot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      c->beta = b;
+      c->rr = rr;
+      c->rrs[slot ^ 1] = rr;
+      c->count++;
+      c->steps++;
+      c->tsteps++;
+      cg_check(c, rr);
+    }
+  }
+  __device__ void elem(int64_t i, double*) { p[i] = (T)(b * (double)p[i] + (double)r[i]); }
+  __device__ void fin(const double*) {}
 };
 
 // p = beta p + r  (blas::Scalar_axy(beta, p, r, p)), optionally twice.
@@ -538,10 +597,17 @@ struct CgLevel : Level {
     OpCgInit<T, 0> op{src, x, r.p, p.p, nullptr, nullptr, ctl(), !x0given, !x0given};
     launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, (x0given ? 24.0 : 32.0) * n);
   }
-  void enqueue_unit(int64_t) override {
-    launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{p.p, q.p, FinAlpha{ctl()}}, gstep(), sizeof(T) * (double)n);
-    launch_map<1>(c, n, OpCgUpdate<T, 0>{x, r.p, p.p, q.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
-    launch_map<0>(c, n, OpXpby<T>{p.p, r.p, nullptr, nullptr, ctl(), 0.0}, gstep(), KS_K_AXPY, 3.0 * sizeof(T) * n);
+  // q = A p with (p, q) partials, then the update and the direction, which
+  // take the totals themselves (OpCgUpdateF, OpXpbyF): three launches per
+  // iteration, no finisher
+  void enqueue_unit(int64_t u) override {
+    const int slot = (int)(u & 1);
+    const int gd = launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{p.p, q.p, FinAlpha{ctl()}}, gstep(),
+                                     sizeof(T) * (double)n, false);
+    const int gu = launch_map_partials<1>(c, n, OpCgUpdateF<T>{x, r.p, p.p, q.p, ctl(), c->part.p, gd, slot, 0.0},
+                                          gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n, RedArgs{c->part2.p});
+    launch_map<0>(c, n, OpXpbyF<T>{p.p, r.p, ctl(), c->part2.p, gu, slot, 0.0}, gstep(), KS_K_AXPY,
+                  3.0 * sizeof(T) * n);
   }
   int64_t max_units() const override { return cfg.maxloop > 0 ? cfg.maxloop : 0; }
   void enqueue_finish() override {
