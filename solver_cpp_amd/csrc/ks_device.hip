@@ -240,15 +240,18 @@ bool build_col_delta(ks_mat* m, const Plan& pl, DBuf<uint16_t>& out, const int32
 
 // The SpMV's 16-bit column indices of A and A^T: int16 offsets when every
 // offset fits (off16), delta-coded columns where they apply, in the order
-// KS_MAT_COL_DELTA asks for; int32 columns otherwise.
+// KS_MAT_COL_DELTA asks for; int32 columns otherwise, and for a side whose
+// SpMV runs on column panels (which read int32 columns).
 void build_index16(ks_mat* m, bool off16) {
   const bool delta_first = (m->flags & KS_MAT_COL_DELTA) != 0;
-  auto side = [&](DBuf<int16_t>& o16, DBuf<uint16_t>& od, const Plan& pl, const int32_t* ptr, const int32_t* col) {
+  auto side = [&](DBuf<int16_t>& o16, DBuf<uint16_t>& od, const Plan& pl, const PanelSet& ps, const int32_t* ptr,
+                  const int32_t* col) {
+    if (ps.np) return;
     if (off16 && !delta_first) build_col16(m, o16, ptr, col);
     else if (!build_col_delta(m, pl, od, ptr, col) && off16) build_col16(m, o16, ptr, col);
   };
-  side(m->col16, m->cold, m->plan, m->ptr.p, m->col.p);
-  if (m->has_t) side(m->tcol16, m->tcold, m->tplan, m->tptr.p, m->tcol.p);
+  side(m->col16, m->cold, m->plan, m->panels, m->ptr.p, m->col.p);
+  if (m->has_t) side(m->tcol16, m->tcold, m->tplan, m->tpanels, m->tptr.p, m->tcol.p);
 }
 
 // Split the CSR (hp, hc: host row pointers and columns of A or A^T) into
@@ -996,6 +999,21 @@ ks_status ks_spmv(ks_mat* m, int32_t trans, int32_t precision, const void* x, vo
   KS_API_CATCH(c)
 }
 
+// A pair of timing events, destroyed on every exit path.
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    KS_HIPCK(hipEventCreate(&a));
+    KS_HIPCK(hipEventCreate(&b));
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+};
+
 ks_status ks_spmv_bench(ks_mat* m, int32_t trans, int32_t precision, const void* x, void* y, int32_t reps,
                         double* ms_per_launch) {
   if (!m || reps <= 0) return KS_ERR_BAD_INPUT;
@@ -1004,22 +1022,18 @@ ks_status ks_spmv_bench(ks_mat* m, int32_t trans, int32_t precision, const void*
   set_device(c);
   KS_CHECK(!trans || m->has_t, KS_ERR_BAD_INPUT, "matrix was created without KS_MAT_TRANSPOSE");
   if (precision == KS_F32) ensure_fp32(m);
-  hipEvent_t a, b;
-  KS_HIPCK(hipEventCreate(&a));
-  KS_HIPCK(hipEventCreate(&b));
-  KS_HIPCK(hipEventRecord(a, c->stream));
+  EventPair ev;
+  KS_HIPCK(hipEventRecord(ev.a, c->stream));
   for (int r = 0; r < reps; ++r) {
     if (precision == KS_F32)
       launch_spmv<float, 0>(c, m->view<float>(trans != 0), (const float*)x, EpiStream<float>{(float*)y}, Gate{});
     else
       launch_spmv<double, 0>(c, m->view<double>(trans != 0), (const double*)x, EpiStream<double>{(double*)y}, Gate{});
   }
-  KS_HIPCK(hipEventRecord(b, c->stream));
-  KS_HIPCK(hipEventSynchronize(b));
+  KS_HIPCK(hipEventRecord(ev.b, c->stream));
+  KS_HIPCK(hipEventSynchronize(ev.b));
   float t = 0;
-  KS_HIPCK(hipEventElapsedTime(&t, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  KS_HIPCK(hipEventElapsedTime(&t, ev.a, ev.b));
   if (ms_per_launch) *ms_per_launch = t / reps;
   return KS_OK;
   KS_API_CATCH(c)
@@ -1037,9 +1051,7 @@ ks_status ks_bench_stream(ks_ctx* c, int32_t kind, int64_t bytes, int32_t reps, 
   DBuf<float> part((size_t)kGrids[3] * (kBlock / 64));
   k_stream_fill<<<grid_map(n16), kBlock, 0, c->stream>>>(n16, x.p, 1.0f);
   KS_HIPCK(hipGetLastError());
-  hipEvent_t a, b;
-  KS_HIPCK(hipEventCreate(&a));
-  KS_HIPCK(hipEventCreate(&b));
+  EventPair ev;
   double best = 0.0;
   for (const int G : kGrids) {
     auto go = [&] {
@@ -1047,18 +1059,16 @@ ks_status ks_bench_stream(ks_ctx* c, int32_t kind, int64_t bytes, int32_t reps, 
       else k_stream_copy<<<G, kBlock, 0, c->stream>>>(n16, x.p, y.p);
     };
     for (int r = 0; r < 2; ++r) go();
-    KS_HIPCK(hipEventRecord(a, c->stream));
+    KS_HIPCK(hipEventRecord(ev.a, c->stream));
     for (int r = 0; r < reps; ++r) go();
-    KS_HIPCK(hipEventRecord(b, c->stream));
-    KS_HIPCK(hipEventSynchronize(b));
+    KS_HIPCK(hipEventRecord(ev.b, c->stream));
+    KS_HIPCK(hipEventSynchronize(ev.b));
     KS_HIPCK(hipGetLastError());
     float t = 0;
-    KS_HIPCK(hipEventElapsedTime(&t, a, b));
+    KS_HIPCK(hipEventElapsedTime(&t, ev.a, ev.b));
     const double moved = (double)bytes * (kind == 1 ? 2 : 1) * reps;
     if (t > 0) best = std::max(best, moved / (t * 1e-3) / 1e9);
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   *gbps = best;
   return KS_OK;
   KS_API_CATCH(c)
