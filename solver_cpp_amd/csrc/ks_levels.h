@@ -186,7 +186,7 @@ struct OpXpby {
 };
 
 // x += a p; r -= a q   (NV 0), or with acc (r,r) and the k-skip test (NV 1).
-template <class T, int NV, int MODE>  // MODE 0: BiCG axpy2, 1: k-skip BiCG last
+template <class T, int NV, int MODE>  // k-skip BiCG: MODE 0 inner steps, 1 the block's last
 struct OpAxpy2 {
   T *x, *r;
   const T *p, *q;
@@ -205,7 +205,8 @@ struct OpAxpy2 {
   }
 };
 
-// BiCG: rs -= a q2; acc0 (rs,r), acc1 (r,r); beta, rr, count++, test.
+// BiCG: rs -= a q2; acc0 (rs,r), acc1 (r,r), whose totals OpXpby2F forms
+// (beta, rr, count++, test).
 template <class T>
 struct OpBicgUpdate {
   T* rs;
@@ -220,14 +221,62 @@ struct OpBicgUpdate {
     acc[0] += (double)v * ri;
     acc[1] += ri * ri;
   }
-  __device__ void fin(const double* s) {
-    c->beta = s[0] / c->rr;
-    c->rr = s[0];
-    c->count++;
-    c->steps++;
-    c->tsteps++;
-    cg_check(c, s[1]);
+  __device__ void fin(const double*) {}
+};
+
+// BiCG's step (bicg.cpp:218-274) with its finishers folded forward as CG's
+// (OpCgUpdateF): the x, r update forms alpha = rr / (p*, A p) from the
+// SpMV's partials, and the direction update forms beta = (r*', r') / (r*, r),
+// the new (r*, r) and the (r, r) stop test from OpBicgUpdate's two partial
+// sums (k_finish<2>'s summation), with rr by iteration parity.
+template <class T>
+struct OpAxpy2F {
+  T *x, *r;
+  const T *p, *q;
+  Ctl* c;
+  const double* dpart;  // the SpMV's (p*, A p) partials
+  int dg, slot;
+  double a;
+  __device__ void prep() {
+    const double dot = partial_totals<1>(dg, dpart)[0];
+    a = c->rrs[slot] / dot;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { c->dotv = dot; c->alpha = a; }
   }
+  __device__ void elem(int64_t i, double*) {
+    x[i] = (T)(a * (double)p[i] + (double)x[i]);
+    r[i] = (T)((-a) * (double)q[i] + (double)r[i]);
+  }
+  __device__ void fin(const double*) {}
+};
+template <class T>
+struct OpXpby2F {
+  T* p;
+  const T* r;
+  T* p2;
+  const T* r2;
+  Ctl* c;
+  const double* upart;  // OpBicgUpdate's ((r*, r), (r, r)) partials
+  int ug, slot;
+  double b;
+  __device__ void prep() {
+    const double* s = partial_totals<2>(ug, upart);
+    const double rz = s[0], rr = s[1];
+    b = rz / c->rrs[slot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      c->beta = b;
+      c->rr = rz;
+      c->rrs[slot ^ 1] = rz;
+      c->count++;
+      c->steps++;
+      c->tsteps++;
+      cg_check(c, rr);
+    }
+  }
+  __device__ void elem(int64_t i, double*) {
+    p[i] = (T)(b * (double)p[i] + (double)r[i]);
+    p2[i] = (T)(b * (double)p2[i] + (double)r2[i]);
+  }
+  __device__ void fin(const double*) {}
 };
 
 // k-skip: x += a p; r -= a Av; p = b p + r; last step of a block adds the
@@ -738,12 +787,18 @@ struct BicgLevel : Level {
     OpCgInit<T, 0> op{src, x, r.p, p.p, rs.p, ps.p, ctl(), !x0given, !x0given};
     launch_map<1>(c, n, op, gouter(), KS_K_REDUCE, 48.0 * n);
   }
-  void enqueue_unit(int64_t) override {
-    launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{ps.p, q.p, FinAlpha{ctl()}}, gstep(), sizeof(T) * (double)n);
-    launch_map<0>(c, n, OpAxpy2<T, 0, 0>{x, r.p, p.p, q.p, ctl(), 0, 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+  // no finisher: OpAxpy2F and OpXpby2F take the totals they need
+  void enqueue_unit(int64_t u) override {
+    const int slot = (int)(u & 1);
+    const int gd = launch_spmv<T, 1>(c, A, p.p, EpiDot<T, FinAlpha>{ps.p, q.p, FinAlpha{ctl()}}, gstep(),
+                                     sizeof(T) * (double)n, false);
+    launch_map<0>(c, n, OpAxpy2F<T>{x, r.p, p.p, q.p, ctl(), c->part.p, gd, slot, 0.0}, gstep(), KS_K_AXPY,
+                  6.0 * sizeof(T) * n);
     launch_spmv<T, 0>(c, AT, ps.p, EpiStore<T>{q.p}, gstep());
-    launch_map<2>(c, n, OpBicgUpdate<T>{rs.p, q.p, r.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 4.0 * sizeof(T) * n);
-    launch_map<0>(c, n, OpXpby<T>{p.p, r.p, ps.p, rs.p, ctl(), 0.0}, gstep(), KS_K_AXPY, 6.0 * sizeof(T) * n);
+    const int gu = launch_map_partials<2>(c, n, OpBicgUpdate<T>{rs.p, q.p, r.p, ctl(), 0.0}, gstep(), KS_K_AXPY,
+                                          4.0 * sizeof(T) * n, red(c));
+    launch_map<0>(c, n, OpXpby2F<T>{p.p, r.p, ps.p, rs.p, ctl(), c->part.p, gu, slot, 0.0}, gstep(), KS_K_AXPY,
+                  6.0 * sizeof(T) * n);
   }
   int64_t max_units() const override { return cfg.maxloop > 0 ? cfg.maxloop : 0; }
   void enqueue_finish() override {
