@@ -87,7 +87,7 @@ struct FinAlpha {
 };
 
 // x += a p; r -= a q; acc (r,r); the finisher counts the step and tests
-// (VPCG, whose beta comes from its own pass).
+// (VPCG and CR, whose beta comes from their own passes).
 template <class T, int MODE>
 struct OpCgUpdate {
   T *x, *r;
