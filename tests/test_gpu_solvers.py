@@ -348,21 +348,32 @@ def test_cli_solver_end_to_end(tmp_path, oracle):
     ("poisson", "vpgmres", "gmres", dict(outer_restart=21, outer_maxloop=2000, outer_eps=1e-10, inner_maxloop=11,
                                          inner_restart=12, inner_eps=0.0, inner_precision=32)),
     ("aniso", "cg", "NO", dict(outer_eps=1e-10, outer_maxloop=20000)),
+    ("poisson", "kskipcg", "NO", dict(outer_eps=1e-10, outer_maxloop=20000, outer_kskip=2)),
+    ("convdiff", "bicg", "NO", dict(outer_eps=1e-10, outer_maxloop=5000)),
+    ("convdiff", "gmres", "NO", dict(outer_eps=1e-10, outer_maxloop=5000, outer_restart=21)),
 ])
 def test_solve_same_in_every_index_format(ctx, mat, outer, inner, kw):
     # a matrix large enough that the SpMV grids fill the GPU (more tiles than
     # resident workgroups): with int16 column offsets, delta-coded 16-bit
-    # columns and int32 columns the solve is the same, bit for bit - same
-    # grids, same partial sums
+    # columns and int32 columns (A and, for BiCG, A^T) the solve is the same,
+    # bit for bit - same grids, same partial sums
     from solver_cpp_amd import api
-    ptr, col, val = P.aniso(740)[:3] if mat == "aniso" else P.poisson2d(740)[:3]
+    if mat == "aniso":
+        ptr, col, val = P.aniso(740)[:3]
+    elif mat == "poisson":
+        ptr, col, val = P.poisson2d(740)[:3]
+    else:
+        ptr, col, val = P.convdiff3d(128)[:3]
     n = len(ptr) - 1
     assert n // 256 > 2048
     b = np.random.default_rng(31).uniform(-1, 1, n)
-    A = api.Matrix(ctx, ptr, col, val, fp32=True, spd=True)
-    C = api.Matrix(ctx, ptr, col, val, fp32=True, spd=True, layout="delta")
-    B = api.Matrix(ctx, ptr, col, val, fp32=True, spd=True, layout="col32")
+    kwm = dict(fp32=True, spd=mat != "convdiff", transpose=mat == "convdiff")
+    A = api.Matrix(ctx, ptr, col, val, **kwm)
+    C = api.Matrix(ctx, ptr, col, val, layout="delta", **kwm)
+    B = api.Matrix(ctx, ptr, col, val, layout="col32", **kwm)
     assert A.index_bytes(False, 32) == 2 and C.index_bytes(False, 32) == 2 and B.index_bytes(False, 32) == 4
+    if mat == "convdiff":
+        assert A.index_bytes(True, 64) == 2 and C.index_bytes(True, 64) == 2 and B.index_bytes(True, 64) == 4
     x, r, h = api.solve(ctx, A, b, outer, inner, **kw)
     z, t, f = api.solve(ctx, C, b, outer, inner, **kw)
     y, s, g = api.solve(ctx, B, b, outer, inner, **kw)
