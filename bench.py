@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--spmv-layouts", default="",
                     help="with --spmv-config: also time these layouts (comma-separated names of "
                          "solver_cpp_amd.api.Matrix.LAYOUTS, e.g. panels_merge,panels_wg,panels_wave)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint timings (forward + gradient per case), one JSON line, no headline: "
+                         "for a profiler run")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB of one plain device copy at the end (counter calibration)")
     return ap.parse_args()
@@ -85,6 +88,11 @@ def main():
     from solver_cpp_amd import api, gen
     from solver_cpp_amd._abi import KERNEL_CLASSES
 
+    if a.adjoint_only:
+        ctx = api.Context(0)
+        print(json.dumps({"suite": "adjoint", "adjoint": adjoint_times(ctx)}), flush=True)
+        ctx.close()
+        return
     if a.spmv_config:
         P = gen.problem(a.spmv_config)
         n, nnz = len(P["ptr"]) - 1, len(P["col"])
