@@ -244,14 +244,19 @@ def test_zero_rhs(ctx):
         assert r["status"] == 0 and np.all(x == 0)
 
 
-def test_x0_given(ctx, oracle):
+@pytest.mark.parametrize("outer", ["cg", "bicg", "gmres"])
+def test_x0_given(ctx, oracle, outer):
     from solver_cpp_amd import api
-    ptr, col, val, b, xt = P.poisson2d(20)
-    A = _mat(ctx, ptr, col, val)
+    ptr, col, val, b, xt = P.poisson2d(20) if outer == "cg" else P.convdiff3d(10)
+    A = _mat(ctx, ptr, col, val, transpose=True)
     x0 = xt + 1e-3 * np.random.default_rng(0).standard_normal(len(xt))
-    x, r, h = api.solve(ctx, A, b, "cg", x0=x0, outer_eps=1e-12)
-    xo, ro, ho = oracle.solve(ptr, col, val, b, "cg", x0=x0, outer_eps=1e-12)
+    kw = dict(outer_eps=1e-12, outer_maxloop=2000, outer_restart=20)
+    x, r, h = api.solve(ctx, A, b, outer, x0=x0, **kw)
+    xo, ro, ho = oracle.solve(ptr, col, val, b, outer, x0=x0, **kw)
     assert r["status"] == 0 and abs(r["iters"] - ro["iters"]) <= 1
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+    m = min(len(h), len(ho))
+    assert m > 0 and np.array_equal(h[:m, 0], ho[:m, 0])
 
 
 def test_adjoint_small(ctx, oracle):
